@@ -1,0 +1,594 @@
+// C-ABI front end of the HIP library (include/opendynamo.h).
+//
+// Owns the device buffers, the compute stream, an IO stream with three
+// pinned staging slots, and one captured hipGraph per RK3 step.  Replaces the
+// orchestration of drunkseal/opendynamo's src/solver.rs (context, NVRTC
+// module, operators, allocations, launch configs, the RK3 loop with its
+// buffer swap and the async D2H of the IO task).
+//
+// Buffer roles over one step (no host-side swap; the state ends in u0):
+//   stage 1: u0 -> u1          traces tr[0] -> tr[1]
+//   stage 2: u1 -> u1 (u^n=u0) traces tr[1] -> tr[2]
+//   stage 3: u1 -> u0 (u^n=u0) traces tr[2] -> tr[0]
+// Each element's work-group reads and writes only its own state entries, so
+// the in-place stage updates are race free; neighbours communicate through
+// the trace and viscous-flux buffers only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/opendynamo.h"
+#include "od_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+od_status fail(od_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(OD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));       \
+    } while (0)
+
+// ---- FR operators for P = 3 (fr_math.rs:3-142), f64 host arithmetic ------
+double legendre(int p, double x) {
+    if (p == 0) return 1.0;
+    if (p == 1) return x;
+    double a = 1.0, b = x, c = 0.0;
+    for (int k = 2; k <= p; ++k) {
+        double kf = (double)k;
+        c = ((2.0 * kf - 1.0) * x * b - (kf - 1.0) * a) / kf;
+        a = b;
+        b = c;
+    }
+    return c;
+}
+double legendre_d(int p, double x) {
+    if (p == 0) return 0.0;
+    double pf = (double)p;
+    return pf / (x * x - 1.0) * (x * legendre(p, x) - legendre(p - 1, x));
+}
+
+struct HostOps {
+    double x[4], D[16], eL[4], eR[4], gL[4], gR[4], w[4];
+};
+
+HostOps fr_operators() {
+    HostOps o{};
+    const int P = 3, n = 4;
+    for (int i = 0; i < n; ++i) {  // Newton on P_4 from cos(pi (i + 3/4) / (n + 1/2))
+        double x = std::cos(M_PI * ((double)i + 0.75) / ((double)n + 0.5));
+        while (true) {
+            double s = legendre(n, x) / legendre_d(n, x);
+            x -= s;
+            if (std::fabs(s) < 1e-14) break;
+        }
+        o.x[i] = x;
+    }
+    std::sort(o.x, o.x + n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            if (i == j) {
+                double s = 0.0;
+                for (int k = 0; k < n; ++k)
+                    if (k != i) s += 1.0 / (o.x[i] - o.x[k]);
+                o.D[i * n + i] = s;
+            } else {
+                double num = 1.0, den = 1.0;
+                for (int k = 0; k < n; ++k)
+                    if (k != i && k != j) num *= o.x[i] - o.x[k];
+                for (int k = 0; k < n; ++k)
+                    if (k != j) den *= o.x[j] - o.x[k];
+                o.D[i * n + j] = num / den;
+            }
+        }
+    for (int i = 0; i < n; ++i) {
+        double l = 1.0, r = 1.0;
+        for (int j = 0; j < n; ++j)
+            if (j != i) {
+                l *= (-1.0 - o.x[j]) / (o.x[i] - o.x[j]);
+                r *= (1.0 - o.x[j]) / (o.x[i] - o.x[j]);
+            }
+        o.eL[i] = l;
+        o.eR[i] = r;
+    }
+    const double sign = (P % 2 == 0) ? 1.0 : -1.0;
+    for (int i = 0; i < n; ++i) {
+        double a = legendre_d(P, o.x[i]), b = legendre_d(P + 1, o.x[i]);
+        o.gR[i] = 0.5 * (a + b);
+        o.gL[i] = 0.5 * sign * (a - b);
+        double dl = legendre_d(n, o.x[i]);
+        o.w[i] = 2.0 / ((1.0 - o.x[i] * o.x[i]) * dl * dl);
+    }
+    return o;
+}
+
+template <class C>
+od::Ops<C> cast_ops(const HostOps& h) {
+    od::Ops<C> o;
+    for (int q = 0; q < 16; ++q) o.D[q] = (C)h.D[q];
+    for (int q = 0; q < 4; ++q) {
+        o.eL[q] = (C)h.eL[q];
+        o.eR[q] = (C)h.eR[q];
+        o.gL[q] = (C)h.gL[q];
+        o.gR[q] = (C)h.gR[q];
+    }
+    return o;
+}
+
+}  // namespace
+
+struct od_ctx {
+    od_params p;
+    int device = 0;
+    hipStream_t stream = nullptr, io_stream = nullptr;
+    int64_t E = 0, NP = 0;
+    size_t ubytes = 0, cbytes = 0;
+    void* u[2] = {nullptr, nullptr};
+    void* tr[3] = {nullptr, nullptr, nullptr};
+    void* fv = nullptr;
+    void* dbg = nullptr;
+    double* sums_partial = nullptr;
+    double* sums_out = nullptr;
+    HostOps ops;
+    bool state_set = false;
+    hipGraphExec_t graph = nullptr;
+    double graph_dt = 0.0;
+    void* pinned[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t slot_done[3] = {nullptr, nullptr, nullptr};
+    uint64_t next_ticket = 1;
+    uint64_t slot_ticket[3] = {0, 0, 0};
+    size_t device_bytes = 0;
+};
+
+namespace {
+
+struct Coeffs {
+    double a, b, cm;
+    bool stage1;
+};
+const Coeffs kStage[3] = {{1.0, 0.0, 1.0, true}, {0.75, 0.25, 0.25, false}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, false}};
+
+template <class UT, class CT>
+od::StageArgs<UT, CT> stage_args(od_ctx* c, int stage, double dt) {
+    od::StageArgs<UT, CT> a{};
+    const od_params& p = c->p;
+    UT* u0 = (UT*)c->u[0];
+    UT* u1 = (UT*)c->u[1];
+    a.uin = stage == 0 ? u0 : u1;
+    a.uprev = u0;
+    a.uout = stage == 2 ? u0 : u1;
+    a.tr_in = (const CT*)c->tr[stage];
+    a.tr_out = (CT*)c->tr[(stage + 1) % 3];
+    a.fv = (CT*)c->fv;
+    a.dbg = (CT*)c->dbg;
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.nz = p.nz;
+    a.wall = p.y_wall ? 1 : 0;
+    a.flux = p.flux_scheme;
+    a.sutherland = p.viscosity_model == 1 ? 1 : 0;
+    a.round_q = (sizeof(CT) == 8 && p.q_real == OD_F32) ? 1 : 0;
+    a.round_rhs = (sizeof(CT) == 8 && p.rhs_real == OD_F32) ? 1 : 0;
+    a.accum_state = p.rk_accum ? 1 : 0;
+    a.stage1 = kStage[stage].stage1 ? 1 : 0;
+    a.mode = 0;
+    // jac = 2 / (L / n) in the compute type (rhs.cu:366-369; rk.cu:93-95)
+    const CT lx = (CT)p.lx, ly = (CT)p.ly, lz = (CT)p.lz;
+    a.jx = ((CT)2.0) / (lx / ((CT)p.nx));
+    a.jy = ((CT)2.0) / (ly / ((CT)p.ny));
+    a.jz = ((CT)2.0) / (lz / ((CT)p.nz));
+    a.mu = (CT)p.mu_ref;
+    a.mu_ref = (CT)p.mu_ref;
+    a.t_ref = (CT)p.t_ref;
+    a.s_suth = (CT)p.s_suth;
+    a.bf = (CT)p.body_force_x;
+    // RK coefficients formed in the compute type from the f32 dt (solver.rs:256, 275-316)
+    const Coeffs& k = kStage[stage];
+    a.ca = (CT)k.a;
+    a.cb = (CT)k.b;
+    a.cc = (CT)k.cm * (CT)dt;
+    a.ua = (UT)k.a;
+    a.ub = (UT)k.b;
+    a.uc = (UT)k.cm * (UT)dt;
+    a.ops = cast_ops<CT>(c->ops);
+    return a;
+}
+
+// kind: 0 = visc, 1 = update, 2 = trace (of u0 into tr[0])
+template <class UT, class CT>
+int launch_kind(od_ctx* c, int kind, int stage, double dt, int mode) {
+    od::StageArgs<UT, CT> a = stage_args<UT, CT>(c, stage, dt);
+    a.mode = mode;
+    if (kind == 2) {
+        a.uin = (const UT*)c->u[0];
+        a.tr_out = (CT*)c->tr[0];
+        return od::launch_trace<UT, CT>(a, c->E, c->stream);
+    }
+    if (kind == 0) return od::launch_visc<UT, CT>(a, c->E, c->stream);
+    if (c->p.viscous) return od::launch_update<UT, CT, true>(a, c->E, c->stream);
+    return od::launch_update<UT, CT, false>(a, c->E, c->stream);
+}
+
+int launch(od_ctx* c, int kind, int stage, double dt, int mode = 0) {
+    const bool uf = c->p.u_real == OD_F32, cf = c->p.compute_real == OD_F32;
+    if (uf && cf) return launch_kind<float, float>(c, kind, stage, dt, mode);
+    if (!uf && !cf) return launch_kind<double, double>(c, kind, stage, dt, mode);
+    if (!uf && cf) return launch_kind<double, float>(c, kind, stage, dt, mode);
+    return launch_kind<float, double>(c, kind, stage, dt, mode);
+}
+
+od_status enqueue_step(od_ctx* c, double dt) {
+    for (int s = 0; s < 3; ++s) {
+        if (c->p.viscous && launch(c, 0, s, dt)) return fail(OD_ERR_HIP, "od_visc_kernel launch failed");
+        if (launch(c, 1, s, dt)) return fail(OD_ERR_HIP, "od_update_kernel launch failed");
+    }
+    return OD_OK;
+}
+
+od_status ensure_graph(od_ctx* c, double dt) {
+    if (c->graph && c->graph_dt == dt) return OD_OK;
+    if (c->graph) {
+        HIP_TRY(hipGraphExecDestroy(c->graph));
+        c->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    od_status s = enqueue_step(c, dt);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (s != OD_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return s;
+    }
+    if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    c->graph_dt = dt;
+    return OD_OK;
+}
+
+bool real_ok(int r) { return r == OD_F32 || r == OD_F64; }
+
+template <class Dst, class Src>
+void convert(Dst* d, const Src* s, size_t n) {
+    for (size_t q = 0; q < n; ++q) d[q] = (Dst)s[q];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* od_last_error(void) { return g_err.c_str(); }
+
+void od_default_params(od_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = p->nz = 1;
+    p->lx = (double)(4.0f * 3.14159265358979323846f);          // config.rs:30-32
+    p->ly = 2.0;                                               // config.rs:33-35
+    p->lz = (double)((4.0f / 3.0f) * 3.14159265358979323846f); // config.rs:36-38
+    p->u_real = p->q_real = p->rhs_real = p->compute_real = OD_F32;
+    p->flux_scheme = OD_FLUX_SLAU2;
+    p->y_wall = 1;
+    p->viscous = 1;
+    p->viscosity_model = 0;
+    p->mu_ref = 0.00016;  // constants.cuh:42, config.rs:77-79
+    p->t_ref = 273.15;
+    p->s_suth = 110.4;
+    p->body_force_x = 0.001;  // rhs.cu:508
+    p->rk_accum = 0;
+    p->fast_math = 1;
+}
+
+od_status od_create(const od_params* p, int device, od_ctx** out) {
+    if (!p || !out) return fail(OD_ERR_INVALID, "od_create: null argument");
+    *out = nullptr;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(OD_ERR_INVALID, "mesh dimensions must be >= 1");
+    if (!real_ok(p->u_real) || !real_ok(p->q_real) || !real_ok(p->rhs_real) || !real_ok(p->compute_real))
+        return fail(OD_ERR_INVALID, "precision types must be OD_F32 or OD_F64");
+    if (p->flux_scheme < 0 || p->flux_scheme > 2) return fail(OD_ERR_INVALID, "flux_scheme must be 0..2");
+    if (p->viscosity_model < 0 || p->viscosity_model > 1) return fail(OD_ERR_INVALID, "viscosity_model must be 0 or 1");
+    if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) return fail(OD_ERR_INVALID, "domain extents must be positive");
+    const int64_t E = (int64_t)p->nx * p->ny * p->nz;
+    if (E > (int64_t)0x7fffffff) return fail(OD_ERR_INVALID, "too many elements for one launch grid");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(OD_ERR_INVALID, "no such HIP device");
+    HIP_TRY(hipSetDevice(device));
+
+    od_ctx* c = new od_ctx();
+    c->p = *p;
+    c->device = device;
+    c->E = E;
+    c->NP = E * 64;
+    c->ubytes = (size_t)p->u_real;
+    c->cbytes = (size_t)p->compute_real;
+    c->ops = fr_operators();
+    auto cleanup = [&](od_status s) {
+        od_destroy(c);
+        return s;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->io_stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(OD_ERR_HIP, "stream creation failed"));
+    const size_t state_bytes = (size_t)5 * c->NP * c->ubytes;
+    const size_t tr_bytes = (size_t)E * od::kTraceStride * c->cbytes;
+    const size_t fv_bytes = (size_t)E * od::kFvStride * c->cbytes;
+    auto dmalloc = [&](void** ptr, size_t bytes) {
+        if (hipMalloc(ptr, bytes) != hipSuccess) return false;
+        c->device_bytes += bytes;
+        return true;
+    };
+    bool ok = dmalloc(&c->u[0], state_bytes) && dmalloc(&c->u[1], state_bytes);
+    for (int q = 0; q < 3 && ok; ++q) ok = dmalloc(&c->tr[q], tr_bytes);
+    ok = ok && (!p->viscous || dmalloc(&c->fv, fv_bytes));
+    ok = ok && dmalloc((void**)&c->sums_partial, (size_t)5 * E * sizeof(double)) &&
+         dmalloc((void**)&c->sums_out, 8 * sizeof(double));
+    if (!ok) return cleanup(fail(OD_ERR_NOMEM, "device allocation failed"));
+    for (int q = 0; q < 3; ++q)
+        if (hipHostMalloc(&c->pinned[q], state_bytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&c->slot_done[q], hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(OD_ERR_NOMEM, "pinned staging allocation failed"));
+    *out = c;
+    return OD_OK;
+}
+
+void od_destroy(od_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->io_stream) (void)hipStreamSynchronize(c->io_stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (void* q : c->u)
+        if (q) (void)hipFree(q);
+    for (void* q : c->tr)
+        if (q) (void)hipFree(q);
+    if (c->fv) (void)hipFree(c->fv);
+    if (c->dbg) (void)hipFree(c->dbg);
+    if (c->sums_partial) (void)hipFree(c->sums_partial);
+    if (c->sums_out) (void)hipFree(c->sums_out);
+    for (int q = 0; q < 3; ++q) {
+        if (c->pinned[q]) (void)hipHostFree(c->pinned[q]);
+        if (c->slot_done[q]) (void)hipEventDestroy(c->slot_done[q]);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->io_stream) (void)hipStreamDestroy(c->io_stream);
+    delete c;
+}
+
+od_status od_set_state(od_ctx* c, const void* host, size_t n, od_real dtype) {
+    if (!c || !host) return fail(OD_ERR_INVALID, "od_set_state: null argument");
+    if (n != (size_t)(5 * c->NP)) return fail(OD_ERR_INVALID, "od_set_state: wrong number of values");
+    if (!real_ok(dtype)) return fail(OD_ERR_INVALID, "od_set_state: bad dtype");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<unsigned char> tmp;
+    const void* src = host;
+    if ((int)dtype != c->p.u_real) {
+        tmp.resize(n * c->ubytes);
+        if (c->p.u_real == OD_F64)
+            convert((double*)tmp.data(), (const float*)host, n);
+        else
+            convert((float*)tmp.data(), (const double*)host, n);
+        src = tmp.data();
+    }
+    HIP_TRY(hipMemcpyAsync(c->u[0], src, n * c->ubytes, hipMemcpyHostToDevice, c->stream));
+    if (launch(c, 2, 0, 0.0)) return fail(OD_ERR_HIP, "od_trace_kernel launch failed");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->state_set = true;
+    return OD_OK;
+}
+
+od_status od_get_state(od_ctx* c, void* host, size_t n, od_real dtype) {
+    if (!c || !host) return fail(OD_ERR_INVALID, "od_get_state: null argument");
+    if (n != (size_t)(5 * c->NP)) return fail(OD_ERR_INVALID, "od_get_state: wrong number of values");
+    if (!real_ok(dtype)) return fail(OD_ERR_INVALID, "od_get_state: bad dtype");
+    HIP_TRY(hipSetDevice(c->device));
+    if ((int)dtype == c->p.u_real) {
+        HIP_TRY(hipMemcpyAsync(host, c->u[0], n * c->ubytes, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return OD_OK;
+    }
+    std::vector<unsigned char> tmp(n * c->ubytes);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), c->u[0], n * c->ubytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->p.u_real == OD_F64)
+        convert((float*)host, (const double*)tmp.data(), n);
+    else
+        convert((double*)host, (const float*)tmp.data(), n);
+    return OD_OK;
+}
+
+od_status od_get_state_async(od_ctx* c, uint64_t* ticket) {
+    if (!c || !ticket) return fail(OD_ERR_INVALID, "od_get_state_async: null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t t = c->next_ticket++;
+    const int slot = (int)(t % 3);
+    // the slot's previous copy must be complete before it is overwritten
+    HIP_TRY(hipEventSynchronize(c->slot_done[slot]));
+    hipEvent_t ready;
+    HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(ready, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->io_stream, ready, 0));
+    HIP_TRY(hipMemcpyAsync(c->pinned[slot], c->u[0], (size_t)5 * c->NP * c->ubytes, hipMemcpyDeviceToHost,
+                           c->io_stream));
+    HIP_TRY(hipEventRecord(c->slot_done[slot], c->io_stream));
+    HIP_TRY(hipEventDestroy(ready));
+    c->slot_ticket[slot] = t;
+    *ticket = t;
+    return OD_OK;
+}
+
+od_status od_wait(od_ctx* c, uint64_t ticket, const void** host_state) {
+    if (!c) return fail(OD_ERR_INVALID, "od_wait: null context");
+    const int slot = (int)(ticket % 3);
+    if (ticket == 0 || c->slot_ticket[slot] != ticket)
+        return fail(OD_ERR_STATE, "od_wait: ticket expired (slot reused) or unknown");
+    HIP_TRY(hipEventSynchronize(c->slot_done[slot]));
+    if (host_state) *host_state = c->pinned[slot];
+    return OD_OK;
+}
+
+od_status od_step(od_ctx* c, double dt, uint32_t n_steps) {
+    if (!c) return fail(OD_ERR_INVALID, "od_step: null context");
+    if (!c->state_set) return fail(OD_ERR_STATE, "od_step: state not set");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(OD_ERR_INVALID, "od_step: dt must be positive and finite");
+    HIP_TRY(hipSetDevice(c->device));
+    od_status s = ensure_graph(c, dt);
+    if (s != OD_OK) return s;
+    for (uint32_t q = 0; q < n_steps; ++q) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    return OD_OK;
+}
+
+od_status od_synchronize(od_ctx* c) {
+    if (!c) return fail(OD_ERR_INVALID, "od_synchronize: null context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->io_stream));
+    return OD_OK;
+}
+
+od_status od_rhs(od_ctx* c, void* host_out, size_t n) {
+    if (!c || !host_out) return fail(OD_ERR_INVALID, "od_rhs: null argument");
+    if (!c->state_set) return fail(OD_ERR_STATE, "od_rhs: state not set");
+    if (n != (size_t)(5 * c->NP)) return fail(OD_ERR_INVALID, "od_rhs: wrong number of values");
+    HIP_TRY(hipSetDevice(c->device));
+    if (!c->dbg) {
+        if (hipMalloc(&c->dbg, (size_t)5 * c->NP * c->cbytes) != hipSuccess)
+            return fail(OD_ERR_NOMEM, "od_rhs: allocation failed");
+        c->device_bytes += (size_t)5 * c->NP * c->cbytes;
+    }
+    // stage-1 operands: uin = u0, tr_in = tr[0]; mode 1 writes the corrected RHS
+    if (c->p.viscous && launch(c, 0, 0, 1.0)) return fail(OD_ERR_HIP, "od_visc_kernel launch failed");
+    if (launch(c, 1, 0, 1.0, 1)) return fail(OD_ERR_HIP, "od_update_kernel launch failed");
+    HIP_TRY(hipMemcpyAsync(host_out, c->dbg, n * c->cbytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return OD_OK;
+}
+
+od_status od_conservation_sums(od_ctx* c, double out[5]) {
+    if (!c || !out) return fail(OD_ERR_INVALID, "od_conservation_sums: null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    double w3[64];
+    for (int pt = 0; pt < 64; ++pt) w3[pt] = c->ops.w[pt & 3] * c->ops.w[(pt >> 2) & 3] * c->ops.w[pt >> 4];
+    const double jac = (c->p.lx / c->p.nx) * (c->p.ly / c->p.ny) * (c->p.lz / c->p.nz) / 8.0;
+    int rc = c->p.u_real == OD_F32
+                 ? od::launch_sums<float>((const float*)c->u[0], c->sums_partial, c->sums_out, c->E, w3, jac, c->stream)
+                 : od::launch_sums<double>((const double*)c->u[0], c->sums_partial, c->sums_out, c->E, w3, jac,
+                                           c->stream);
+    if (rc) return fail(OD_ERR_HIP, "conservation sum kernels failed to launch");
+    HIP_TRY(hipMemcpyAsync(out, c->sums_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return OD_OK;
+}
+
+od_status od_cfl_dt(od_ctx* c, double cfl, double* dt_out) {
+    if (!c || !dt_out) return fail(OD_ERR_INVALID, "od_cfl_dt: null argument");
+    if (!(cfl > 0.0)) return fail(OD_ERR_INVALID, "od_cfl_dt: cfl must be positive");
+    const size_t n = (size_t)5 * c->NP;
+    std::vector<double> h(n);
+    od_status s = od_get_state(c, h.data(), n, OD_F64);
+    if (s != OD_OK) return s;
+    const int64_t NP = c->NP;
+    const double dx = c->p.lx / c->p.nx, dy = c->p.ly / c->p.ny, dz = c->p.lz / c->p.nz;
+    double lam = 0.0;
+    for (int64_t q = 0; q < NP; ++q) {
+        double r = h[q], ru = h[NP + q], rv = h[2 * NP + q], rw = h[3 * NP + q], e = h[4 * NP + q];
+        double ke = 0.5 * (ru * ru + rv * rv + rw * rw) / r;
+        double pp = std::fmax(1e-7, 0.4 * (e - ke));
+        double cs = std::sqrt(1.4 * pp / r);
+        double l = (std::fabs(ru / r) + cs) / dx + (std::fabs(rv / r) + cs) / dy + (std::fabs(rw / r) + cs) / dz;
+        lam = std::fmax(lam, l);
+    }
+    *dt_out = (double)(float)(cfl / (7.0 * lam));
+    return OD_OK;
+}
+
+od_status od_time_steps(od_ctx* c, double dt, uint32_t n_steps, double* ms_total) {
+    if (!c || !ms_total) return fail(OD_ERR_INVALID, "od_time_steps: null argument");
+    if (!c->state_set) return fail(OD_ERR_STATE, "od_time_steps: state not set");
+    HIP_TRY(hipSetDevice(c->device));
+    od_status s = ensure_graph(c, dt);
+    if (s != OD_OK) return s;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, c->stream));
+    for (uint32_t q = 0; q < n_steps; ++q) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    HIP_TRY(hipEventRecord(b, c->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    HIP_TRY(hipEventDestroy(a));
+    HIP_TRY(hipEventDestroy(b));
+    *ms_total = ms;
+    return OD_OK;
+}
+
+od_status od_time_kernels(od_ctx* c, double dt, uint32_t n_steps, double out_ms[4], const char* names[4],
+                          int* n_kinds) {
+    if (!c || !out_ms) return fail(OD_ERR_INVALID, "od_time_kernels: null argument");
+    if (!c->state_set) return fail(OD_ERR_STATE, "od_time_kernels: state not set");
+    HIP_TRY(hipSetDevice(c->device));
+    const int kinds = 2;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind_of;
+    auto rec = [&](int kind) -> od_status {
+        hipEvent_t a, b;
+        HIP_TRY(hipEventCreate(&a));
+        HIP_TRY(hipEventCreate(&b));
+        ev.push_back(a);
+        ev.push_back(b);
+        kind_of.push_back(kind);
+        return OD_OK;
+    };
+    for (uint32_t q = 0; q < n_steps; ++q)
+        for (int s = 0; s < 3; ++s)
+            for (int kind = 0; kind < kinds; ++kind) {
+                if (kind == 0 && !c->p.viscous) continue;
+                od_status st = rec(kind);
+                if (st != OD_OK) return st;
+                HIP_TRY(hipEventRecord(ev[ev.size() - 2], c->stream));
+                if (launch(c, kind, s, dt)) return fail(OD_ERR_HIP, "kernel launch failed");
+                HIP_TRY(hipEventRecord(ev.back(), c->stream));
+            }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double sum[4] = {0, 0, 0, 0};
+    int cnt[4] = {0, 0, 0, 0};
+    for (size_t q = 0; q < kind_of.size(); ++q) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]));
+        sum[kind_of[q]] += ms;
+        cnt[kind_of[q]] += 1;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 4; ++k) {
+        out_ms[k] = (k < kinds && cnt[k]) ? sum[k] / cnt[k] : 0.0;
+        if (names) names[k] = k < kinds ? od::kernel_name(k) : "";
+    }
+    if (n_kinds) *n_kinds = kinds;
+    return OD_OK;
+}
+
+od_status od_info(od_ctx* c, double out[4]) {
+    if (!c || !out) return fail(OD_ERR_INVALID, "od_info: null argument");
+    out[0] = (double)c->NP;
+    out[1] = (double)c->device_bytes;
+    out[2] = c->p.viscous ? 6.0 : 3.0;
+    out[3] = 0.0;
+    return OD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Internal interface between the C-ABI front end (od_api.hip) and the
+// gfx950 kernels (od_kernels.hip).
+//
+// Device data layout (all arrays in HBM, element e = ex + nx*ey + nx*ny*ez):
+//   state   [5][E*64]        UT   var-major, pt = i + 4j + 16k (file layout)
+//   traces  [E][8][5][16]    CT   face traces of the state, per element:
+//            0 xL, 1 xR, 2 yL, 3 yR  pairwise (t0+t1)+(t2+t3)
+//            4 zL, 5 zR              sequential from 0
+//            6 xL, 7 xR              sequential from the first product
+//           (face point fp: x j+4k, y i+4k, z i+4j)
+//   fv      [E][6][4][16]    CT   viscous flux components 1..4 at the
+//                                 element's own faces xL,xR,yL,yR,zL,zR
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace od {
+
+constexpr int kTraceFaces = 8;
+constexpr int kTraceStride = kTraceFaces * 5 * 16;  // per element
+constexpr int kFvStride = 6 * 4 * 16;               // per element
+
+template <class C>
+struct Ops {
+    C D[16];   // D[i][m] = l_m'(x_i)
+    C eL[4];   // l_i(-1)
+    C eR[4];   // l_i(+1)
+    C gL[4];   // g_left'(x_i)
+    C gR[4];   // g_right'(x_i)
+};
+
+template <class UT, class CT>
+struct StageArgs {
+    const UT* uin;     // state the stage differentiates
+    const UT* uprev;   // u^n (stage 1: == uin)
+    UT* uout;          // updated state (may alias uin or uprev per element)
+    const CT* tr_in;   // traces of uin
+    CT* tr_out;        // traces of uout
+    CT* fv;            // face viscous fluxes of uin
+    CT* dbg;           // mode 1: corrected RHS output [5][N]
+    int32_t nx, ny, nz;
+    int32_t wall, flux, sutherland, round_q, round_rhs, accum_state, stage1, mode;
+    CT jx, jy, jz;
+    CT mu, mu_ref, t_ref, s_suth, bf;
+    CT ca, cb, cc;     // blend coefficients in the compute type
+    UT ua, ub, uc;     // blend coefficients in the state type
+    Ops<CT> ops;
+};
+
+// Kernel launchers (explicitly instantiated for UT, CT in {float, double}).
+template <class UT, class CT>
+int launch_trace(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+template <class UT, class CT>
+int launch_visc(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+template <class UT, class CT, bool VISC>
+int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+template <class UT>
+int launch_sums(const UT* u, double* partial, double* out, int64_t n_elem, const double w3[64],
+                double jac, void* stream);
+
+const char* kernel_name(int kind);
+
+}  // namespace od

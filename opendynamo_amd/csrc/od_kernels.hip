@@ -1,0 +1,723 @@
+// gfx950 kernels of the FR P=3 compressible Navier-Stokes RK3 step.
+//
+// Per RK stage (drunkseal/opendynamo: solver.rs:610-738, rhs.cu, rk.cu):
+//   od_visc_kernel    one wave64 per element: BR1-corrected gradient from the
+//                     element's LDS tile and its neighbours' stored face
+//                     traces, gradient face extrapolation, viscous normal flux
+//                     at the element's 6 faces -> fv  (replaces the gradient
+//                     half of compute_rhs_volume_kernel and the viscous half of
+//                     the face kernels)
+//   od_update_kernel  one wave64 per element: gradient again (register only),
+//                     volume flux divergence, interface fluxes at all 6 own
+//                     faces (SLAU2 / AUSM+ / Rusanov, slip wall), FR
+//                     correction, SSP-RK3 blend, face traces of the new state
+//                     (replaces the rest of the volume kernel, the x/y/wall/z
+//                     face kernels, the j arrays and rk_stage*_kernel)
+// Each face flux is evaluated by both elements that share the face, from the
+// same operands in the same order, so both sides see identical bits and no
+// face array, atomic or second pass is needed.
+//
+// Lane = solution point pt = i + 4j + 16k of the element.  Line contractions
+// stage through LDS.  The arithmetic of every quantity (association order,
+// no contraction: built with -ffp-contract=off) is that of the reference
+// kernel that computes it; comments cite the reference line.
+#include <hip/hip_runtime.h>
+
+#include "od_internal.hpp"
+
+namespace od {
+namespace {
+
+template <class T>
+__device__ __forceinline__ T sel4(T a0, T a1, T a2, T a3, int i) {
+    return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
+template <class T>
+__device__ __forceinline__ T sel3(T a0, T a1, T a2, int d) {
+    return d == 0 ? a0 : (d == 1 ? a1 : a2);
+}
+
+// ---------------------------------------------------------------------------
+// Point physics in the compute type C (device_functions.cuh:1-257).
+// Direction d = 0/1/2 is an argument; all three reference variants
+// (compute_euler_flux_{x,y,z}, compute_viscous_flux_{x,y,z}) are the same
+// expression tree with the direction's operands, which is what the selects
+// pick, so the result is bitwise that of the matching variant.
+// ---------------------------------------------------------------------------
+template <class C>
+__device__ __forceinline__ C gamma_m1() { return ((C)1.4) - ((C)1.0); }
+
+template <class C>
+__device__ __forceinline__ C pressure(C rho, C ru, C rv, C rw, C e) {  // :1-5
+    C ke = ((C)0.5) * (ru * ru + rv * rv + rw * rw) / rho;
+    C p = gamma_m1<C>() * (e - ke);
+    return fmax(((C)1e-7), p);
+}
+
+template <class C>
+__device__ __forceinline__ void euler_flux(int d, const C* s, C* f) {  // :7-21
+    C rho = s[0], e = s[4];
+    C p = pressure(rho, s[1], s[2], s[3], e);
+    C md = sel3(s[1], s[2], s[3], d);
+    C ud = md / rho;
+    f[0] = md;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        C t = s[1 + c] * ud;
+        f[1 + c] = (c == d) ? (t + p) : t;
+    }
+    f[4] = ud * (e + p);
+}
+
+// :23-102.  g[3*var + dir]; G[a][b] = d(u_a)/d(x_b).
+template <class C>
+__device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu, C* f) {
+    C rho = s[0];
+    C u[3] = {s[1] / rho, s[2] / rho, s[3] / rho};
+    C G[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) G[a][b] = (g[3 * (a + 1) + b] - u[a] * g[b]) / rho;
+    C div = G[0][0] + G[1][1] + G[2][2];
+    const C two = (C)2.0, tt = ((C)2.0) / ((C)3.0);
+    C t00 = mu * (two * G[0][0] - tt * div);
+    C t11 = mu * (two * G[1][1] - tt * div);
+    C t22 = mu * (two * G[2][2] - tt * div);
+    C t01 = mu * (G[0][1] + G[1][0]);
+    C t02 = mu * (G[0][2] + G[2][0]);
+    C t12 = mu * (G[1][2] + G[2][1]);
+    C r0 = sel3(t00, t01, t02, d), r1 = sel3(t01, t11, t12, d), r2 = sel3(t02, t12, t22, d);
+    C ke = ((C)0.5) * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / rho;
+    C p = gamma_m1<C>() * (s[4] - ke);
+    C gd = sel3(g[0], g[1], g[2], d);
+    C gu = sel3(g[3], g[4], g[5], d);
+    C gv = sel3(g[6], g[7], g[8], d);
+    C gw = sel3(g[9], g[10], g[11], d);
+    C ge = sel3(g[12], g[13], g[14], d);
+    C dke = (s[1] * gu + s[2] * gv + s[3] * gw) / rho - ke * gd / rho;
+    C dp = gamma_m1<C>() * (ge - dke);
+    C coeff = mu * ((C)1.4) / (((C)0.72) * gamma_m1<C>());
+    C q = -coeff * (dp / rho - p * gd / (rho * rho));
+    f[0] = (C)0.0;
+    f[1] = r0;
+    f[2] = r1;
+    f[3] = r2;
+    f[4] = u[0] * r0 + u[1] * r1 + u[2] * r2 - q;
+}
+
+template <class C>
+__device__ __forceinline__ C sound_speed(C p, C rho) {
+    return sqrt(fmax(((C)0.0), ((C)1.4) * p / fmax(((C)1e-7), rho)));
+}
+
+template <class C>
+__device__ __forceinline__ void rusanov(const C* l, const C* r, const C* n, C* f) {  // :104-128
+    C ul = l[1] / l[0], vl = l[2] / l[0], wl = l[3] / l[0];
+    C ur = r[1] / r[0], vr = r[2] / r[0], wr = r[3] / r[0];
+    C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
+    C pr = pressure(r[0], r[1], r[2], r[3], r[4]);
+    C vnl = ul * n[0] + vl * n[1] + wl * n[2];
+    C vnr = ur * n[0] + vr * n[1] + wr * n[2];
+    C cl = sound_speed(pl, l[0]), cr = sound_speed(pr, r[0]);
+    C lam = fmax(fabs(vnl) + cl, fabs(vnr) + cr);
+    C fl[5], fr[5];
+    fl[0] = l[0] * vnl;
+    fl[1] = l[1] * vnl + pl * n[0];
+    fl[2] = l[2] * vnl + pl * n[1];
+    fl[3] = l[3] * vnl + pl * n[2];
+    fl[4] = vnl * (l[4] + pl);
+    fr[0] = r[0] * vnr;
+    fr[1] = r[1] * vnr + pr * n[0];
+    fr[2] = r[2] * vnr + pr * n[1];
+    fr[3] = r[3] * vnr + pr * n[2];
+    fr[4] = vnr * (r[4] + pr);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) f[v] = ((C)0.5) * (fl[v] + fr[v] - lam * (r[v] - l[v]));
+}
+
+template <class C>
+__device__ __forceinline__ void ausm_plus(const C* l, const C* r, const C* n, C* f) {  // :130-188
+    C ul = l[1] / l[0], vl = l[2] / l[0], wl = l[3] / l[0];
+    C ur = r[1] / r[0], vr = r[2] / r[0], wr = r[3] / r[0];
+    C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
+    C pr = pressure(r[0], r[1], r[2], r[3], r[4]);
+    C vnl = ul * n[0] + vl * n[1] + wl * n[2];
+    C vnr = ur * n[0] + vr * n[1] + wr * n[2];
+    C hl = (l[4] + pl) / l[0], hr = (r[4] + pr) / r[0];
+    C cl = sound_speed(pl, l[0]), cr = sound_speed(pr, r[0]);
+    C cm = fmax(cl, cr);
+    C ml = vnl / cm, mr = vnr / cm;
+    const C one = (C)1.0, q = (C)0.25, alpha = (C)0.1875, beta = (C)0.125, zero = (C)0.0;
+    C mp, mm, pp, pm;
+    if (fabs(ml) >= one) {
+        mp = ((C)0.5) * (ml + fabs(ml));
+        pp = (ml > zero) ? one : zero;
+    } else {
+        mp = q * (ml + one) * (ml + one) + beta * (ml * ml - one) * (ml * ml - one);
+        pp = q * (ml + one) * (ml + one) * (((C)2.0) - ml) + alpha * ml * (ml * ml - one) * (ml * ml - one);
+    }
+    if (fabs(mr) >= one) {
+        mm = ((C)0.5) * (mr - fabs(mr));
+        pm = (mr < zero) ? one : zero;
+    } else {
+        mm = -q * (mr - one) * (mr - one) - beta * (mr * mr - one) * (mr * mr - one);
+        pm = q * (mr - one) * (mr - one) * (((C)2.0) + mr) - alpha * mr * (mr * mr - one) * (mr * mr - one);
+    }
+    C mach = mp + mm;
+    C p = pp * pl + pm * pr;
+    bool pos = mach > zero;
+    C mf = mach * cm * (pos ? l[0] : r[0]);
+    C us = pos ? ul : ur, vs = pos ? vl : vr, ws = pos ? wl : wr, hs = pos ? hl : hr;
+    f[0] = mf;
+    f[1] = mf * us + p * n[0];
+    f[2] = mf * vs + p * n[1];
+    f[3] = mf * ws + p * n[2];
+    f[4] = mf * hs;
+}
+
+template <class C>
+__device__ __forceinline__ void slau2(const C* l, const C* r, const C* n, C* f) {  // :190-257
+    C ul = l[1] / l[0], vl = l[2] / l[0], wl = l[3] / l[0];
+    C ur = r[1] / r[0], vr = r[2] / r[0], wr = r[3] / r[0];
+    C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
+    C pr = pressure(r[0], r[1], r[2], r[3], r[4]);
+    C vnl = ul * n[0] + vl * n[1] + wl * n[2];
+    C vnr = ur * n[0] + vr * n[1] + wr * n[2];
+    C hl = (l[4] + pl) / l[0], hr = (r[4] + pr) / r[0];
+    C sql = ul * ul + vl * vl + wl * wl;
+    C sqr = ur * ur + vr * vr + wr * wr;
+    C cl = sound_speed(pl, l[0]), cr = sound_speed(pr, r[0]);
+    const C half = (C)0.5, one = (C)1.0, zero = (C)0.0;
+    C cf = half * (cl + cr);
+    C ml = vnl / cf, mr = vnr / cf;
+    C mt = fmin(one, (one / cf) * sqrt(half * (sql + sqr)));
+    C chi = one - mt;
+    chi = chi * chi;
+    C g = -fmax(fmin(ml, zero), -one) * fmin(fmax(mr, zero), one);
+    C vavg = (l[0] * fabs(vnl) + r[0] * fabs(vnr)) / (l[0] + r[0]);
+    C val = (one - g) * vavg + g * fabs(vnl);
+    C var = (one - g) * vavg + g * fabs(vnr);
+    C mf = half * (l[0] * (vnl + val) + r[0] * (vnr - var) - (chi / cf) * (pr - pl));
+    C bl = (fabs(ml) < one) ? ((C)0.25) * (((C)2.0) - ml) * (ml + one) * (ml + one)
+                            : ((ml >= zero) ? one : zero);
+    C br = (fabs(mr) < one) ? ((C)0.25) * (((C)2.0) + mr) * (mr - one) * (mr - one)
+                            : ((mr >= zero) ? zero : one);
+    C pf = half * (pl + pr) + half * (bl - br) * (pl - pr);
+    pf += sqrt(half * (sql + sqr)) * (bl + br - one) * cf * half * (l[0] + r[0]);
+    bool pos = mf > zero;
+    C us = pos ? ul : ur, vs = pos ? vl : vr, ws = pos ? wl : wr, hs = pos ? hl : hr;
+    f[0] = mf;
+    f[1] = mf * us + pf * n[0];
+    f[2] = mf * vs + pf * n[1];
+    f[3] = mf * ws + pf * n[2];
+    f[4] = mf * hs;
+}
+
+template <class C>
+__device__ __forceinline__ void interface_flux(int scheme, const C* l, const C* r, const C* n, C* f) {
+    if (scheme == 0)
+        slau2(l, r, n, f);
+    else if (scheme == 1)
+        ausm_plus(l, r, n, f);
+    else
+        rusanov(l, r, n, f);
+}
+
+// Sutherland law, theta = gamma p / rho (no reference counterpart; the
+// reference parses the keys but uses a constant mu, constants.cuh:42).
+template <class C>
+__device__ __forceinline__ C sutherland(const C* s, C mu_ref, C t_ref, C s_suth) {
+    C theta = ((C)1.4) * pressure(s[0], s[1], s[2], s[3], s[4]) / s[0];
+    return ((mu_ref * (theta * sqrt(theta))) * (t_ref + s_suth)) / (t_ref * theta + s_suth);
+}
+
+// ---------------------------------------------------------------------------
+// Element geometry helpers.
+// ---------------------------------------------------------------------------
+struct Elem {
+    int64_t e;
+    int ex, ey, ez;
+    int64_t nb[6];  // xL, xR, yL, yR, zL, zR neighbour; -1 = slip wall
+};
+
+__device__ __forceinline__ Elem make_elem(int64_t e, int nx, int ny, int nz, int wall) {
+    Elem r;
+    r.e = e;
+    r.ex = (int)(e % nx);
+    r.ey = (int)((e / nx) % ny);
+    r.ez = (int)(e / ((int64_t)nx * ny));
+    auto id = [&](int x, int y, int z) { return (int64_t)x + (int64_t)nx * (y + (int64_t)ny * z); };
+    r.nb[0] = id((r.ex - 1 + nx) % nx, r.ey, r.ez);
+    r.nb[1] = id((r.ex + 1) % nx, r.ey, r.ez);
+    if (wall) {
+        r.nb[2] = r.ey > 0 ? id(r.ex, r.ey - 1, r.ez) : -1;
+        r.nb[3] = r.ey < ny - 1 ? id(r.ex, r.ey + 1, r.ez) : -1;
+    } else {
+        r.nb[2] = id(r.ex, (r.ey - 1 + ny) % ny, r.ez);
+        r.nb[3] = id(r.ex, (r.ey + 1) % ny, r.ez);
+    }
+    r.nb[4] = id(r.ex, r.ey, (r.ez - 1 + nz) % nz);
+    r.nb[5] = id(r.ex, r.ey, (r.ez + 1) % nz);
+    return r;
+}
+
+// Points of the line through face point fp of a face normal to d:
+// x: fp = j+4k, y: fp = i+4k, z: fp = i+4j.
+__device__ __forceinline__ void face_line(int d, int fp, int& base, int& stride) {
+    base = d == 0 ? 4 * fp : (d == 1 ? (fp & 3) + 16 * (fp >> 2) : fp);
+    stride = d == 0 ? 1 : (d == 1 ? 4 : 16);
+}
+
+// Trace orders of the reference: 0 = pairwise (t0+t1)+(t2+t3) (shuffle
+// butterfly, rhs.cu:68-85, 595-599, 665-674), 1 = sequential from 0
+// (rhs.cu:89-101, 171-176, 716-727), 2 = sequential from the first product
+// (rhs.cu:131-138, 540-543).
+template <class C>
+__device__ __forceinline__ C trace4(const C* q, int base, int stride, C c0, C c1, C c2, C c3, int order) {
+    C t0 = q[base] * c0, t1 = q[base + stride] * c1, t2 = q[base + 2 * stride] * c2, t3 = q[base + 3 * stride] * c3;
+    C pw = (t0 + t1) + (t2 + t3);
+    C s0 = (C)0;
+    s0 += t0;
+    s0 += t1;
+    s0 += t2;
+    s0 += t3;
+    C s1 = t0 + t1 + t2 + t3;
+    return order == 0 ? pw : (order == 1 ? s0 : s1);
+}
+
+// Riemann-input / gradient-slice order of the face normal to d (x: seq from
+// first, y: pairwise, z: seq from 0).
+__device__ __forceinline__ int face_order(int d) { return d == 0 ? 2 : (d == 1 ? 0 : 1); }
+
+template <class C>
+__device__ __forceinline__ void accumulate_jump(C& gc, C own, C nb, C gp, C jac) {  // rhs.cu:105-108
+    C jmp = ((C)0.5) * (nb + own) - own;
+    gc += jmp * gp * jac;
+}
+
+// BR1-corrected gradient at the lane's point (rhs.cu:366-465), Q round trip
+// included.  sU: the element's state in LDS; neighbour traces from tr_in.
+template <class UT, class CT>
+__device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const CT (*sU)[64], const Elem& el,
+                                              int i, int j, int k, CT* g) {
+    const Ops<CT>& o = a.ops;
+    CT Di[4], Dj[4], Dk[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        Di[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], i);
+        Dj[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], j);
+        Dk[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], k);
+    }
+    const int xb = 4 * j + 16 * k, yb = i + 16 * k, zb = i + 4 * j;
+    const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
+    CT gx[5], gy[5], gz[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT dz = 0, dxi = 0, deta = 0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) dz += Dk[m] * sU[v][zb + 16 * m];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) dxi += Di[m] * sU[v][xb + m];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) deta += Dj[m] * sU[v][yb + 4 * m];
+        gx[v] = dxi * a.jx;
+        gy[v] = deta * a.jy;
+        gz[v] = dz * a.jz;
+    }
+    const CT gLi = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], i), gRi = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], i);
+    const CT gLj = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], j), gRj = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], j);
+    const CT gLk = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], k), gRk = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], k);
+    const CT* tr = a.tr_in;
+    // x faces: own and neighbour traces pairwise (rhs.cu:385-399)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT own_l = trace4(sU[v], xb, 1, o.eL[0], o.eL[1], o.eL[2], o.eL[3], 0);
+        CT own_r = trace4(sU[v], xb, 1, o.eR[0], o.eR[1], o.eR[2], o.eR[3], 0);
+        CT nb_l = tr[el.nb[0] * kTraceStride + 1 * 80 + v * 16 + fpx];
+        CT nb_r = tr[el.nb[1] * kTraceStride + 0 * 80 + v * 16 + fpx];
+        accumulate_jump(gx[v], own_l, nb_l, gLi, a.jx);
+        accumulate_jump(gx[v], own_r, nb_r, gRi, a.jx);
+    }
+    // y faces, slip-wall ghost negates the momenta (rhs.cu:401-438, 111-114)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT own_l = trace4(sU[v], yb, 4, o.eL[0], o.eL[1], o.eL[2], o.eL[3], 0);
+        CT own_r = trace4(sU[v], yb, 4, o.eR[0], o.eR[1], o.eR[2], o.eR[3], 0);
+        const CT sign = (v > 0 && v < 4) ? ((CT)-1.0) : ((CT)1.0);
+        CT nb_l = el.nb[2] >= 0 ? tr[el.nb[2] * kTraceStride + 3 * 80 + v * 16 + fpy] : own_l * sign;
+        CT nb_r = el.nb[3] >= 0 ? tr[el.nb[3] * kTraceStride + 2 * 80 + v * 16 + fpy] : own_r * sign;
+        accumulate_jump(gy[v], own_l, nb_l, gLj, a.jy);
+        accumulate_jump(gy[v], own_r, nb_r, gRj, a.jy);
+    }
+    // z faces: sequential sums from 0 (rhs.cu:440-455)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT own_l = trace4(sU[v], zb, 16, o.eL[0], o.eL[1], o.eL[2], o.eL[3], 1);
+        CT own_r = trace4(sU[v], zb, 16, o.eR[0], o.eR[1], o.eR[2], o.eR[3], 1);
+        CT nb_l = tr[el.nb[4] * kTraceStride + 5 * 80 + v * 16 + fpz];
+        CT nb_r = tr[el.nb[5] * kTraceStride + 4 * 80 + v * 16 + fpz];
+        accumulate_jump(gz[v], own_l, nb_l, gLk, a.jz);
+        accumulate_jump(gz[v], own_r, nb_r, gRk, a.jz);
+    }
+    // Q_REAL round trip (rhs.cu:457-461): only narrows when CT is double.
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT x = gx[v], y = gy[v], z = gz[v];
+        if (a.round_q) {
+            x = (CT)(float)x;
+            y = (CT)(float)y;
+            z = (CT)(float)z;
+        }
+        g[3 * v + 0] = x;
+        g[3 * v + 1] = y;
+        g[3 * v + 2] = z;
+    }
+}
+
+// Face traces of the state in sU -> tr_out (layout in od_internal.hpp).
+template <class CT>
+__device__ __forceinline__ void write_traces(const Ops<CT>& o, const CT (*sU)[64], CT* tr_out, int64_t e, int lane,
+                                             bool need_pairwise_x) {
+    for (int task = lane; task < 128; task += 64) {
+        int f = task >> 4, fp = task & 15;
+        if (!need_pairwise_x && f < 2) continue;
+        int d = f < 6 ? (f >> 1) : 0;
+        int order = f < 4 ? 0 : (f < 6 ? 1 : 2);
+        int base, stride;
+        face_line(d, fp, base, stride);
+        bool right = f & 1;
+        CT c0 = right ? o.eR[0] : o.eL[0], c1 = right ? o.eR[1] : o.eL[1];
+        CT c2 = right ? o.eR[2] : o.eL[2], c3 = right ? o.eR[3] : o.eL[3];
+        CT* dst = tr_out + e * kTraceStride + f * 80 + fp;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) dst[v * 16] = trace4(sU[v], base, stride, c0, c1, c2, c3, order);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels.
+// ---------------------------------------------------------------------------
+template <class UT, class CT>
+__global__ __launch_bounds__(64) void od_trace_kernel(StageArgs<UT, CT> a) {
+    __shared__ CT sU[5][64];
+    const int pt = threadIdx.x;
+    const int64_t e = blockIdx.x;
+    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sU[v][pt] = (CT)a.uin[v * NP + e * 64 + pt];
+    __syncthreads();
+    write_traces(a.ops, sU, a.tr_out, e, pt, true);
+}
+
+template <class UT, class CT>
+__global__ __launch_bounds__(64) void od_visc_kernel(StageArgs<UT, CT> a) {
+    __shared__ CT sU[5][64];
+    __shared__ CT sG[15][64];
+    const int pt = threadIdx.x;
+    const int64_t e = blockIdx.x;
+    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
+    const Elem el = make_elem(e, a.nx, a.ny, a.nz, a.wall);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sU[v][pt] = (CT)a.uin[v * NP + e * 64 + pt];
+    __syncthreads();
+    CT g[15];
+    lane_gradient(a, sU, el, i, j, k, g);
+#pragma unroll
+    for (int c = 0; c < 15; ++c) sG[c][pt] = g[c];
+    __syncthreads();
+    const Ops<CT>& o = a.ops;
+    // viscous normal flux at the 6 own faces, 16 points each (the
+    // dispatch_viscous_flux calls of compute_face_j / compute_face_j_wall,
+    // rhs.cu:284-285, 318), from the Riemann-order state trace and the
+    // gradient slice (rhs.cu:126-180, 540-543, 663-674, 716-727).
+    for (int task = pt; task < 96; task += 64) {
+        int f = task >> 4, fp = task & 15, d = f >> 1;
+        int base, stride;
+        face_line(d, fp, base, stride);
+        int order = face_order(d);
+        bool right = f & 1;
+        CT c0 = right ? o.eR[0] : o.eL[0], c1 = right ? o.eR[1] : o.eL[1];
+        CT c2 = right ? o.eR[2] : o.eL[2], c3 = right ? o.eR[3] : o.eL[3];
+        CT st[5], gt[15], fvv[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) st[v] = trace4(sU[v], base, stride, c0, c1, c2, c3, order);
+#pragma unroll
+        for (int c = 0; c < 15; ++c) gt[c] = trace4(sG[c], base, stride, c0, c1, c2, c3, order);
+        CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+        viscous_flux(d, st, gt, mu, fvv);
+        CT* dst = a.fv + e * kFvStride + f * 64 + fp;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dst[c * 16] = fvv[1 + c];
+    }
+}
+
+template <class UT, class CT, bool VISC>
+__global__ __launch_bounds__(64) void od_update_kernel(StageArgs<UT, CT> a) {
+    __shared__ CT sU[5][64];
+    __shared__ CT sF[5][64];
+    __shared__ CT sJ[6][5][16];
+    const int pt = threadIdx.x;
+    const int64_t e = blockIdx.x;
+    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
+    const Elem el = make_elem(e, a.nx, a.ny, a.nz, a.wall);
+    const Ops<CT>& o = a.ops;
+
+    UT ucur[5], uprv[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        ucur[v] = a.uin[v * NP + e * 64 + pt];
+        uprv[v] = a.uprev[v * NP + e * 64 + pt];
+        sU[v][pt] = (CT)ucur[v];
+    }
+    __syncthreads();
+
+    CT g[15];
+    if (VISC) lane_gradient(a, sU, el, i, j, k, g);
+    CT st[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) st[v] = sU[v][pt];
+    CT mu = (CT)0;
+    if (VISC) mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+
+    // Volume flux divergence, x then y then z (rhs.cu:229-266, 490-510).
+    CT rs[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        CT fe[5], F[5];
+        euler_flux(d, st, fe);
+        if (VISC) {
+            CT fvv[5];
+            viscous_flux(d, st, g, mu, fvv);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) F[v] = fe[v] - fvv[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) F[v] = fe[v];
+        }
+        if (d > 0) __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sF[v][pt] = F[v];
+        __syncthreads();
+        const int idx = d == 0 ? i : (d == 1 ? j : k);
+        const int base = d == 0 ? 4 * j + 16 * k : (d == 1 ? i + 16 * k : i + 4 * j);
+        const int stride = d == 0 ? 1 : (d == 1 ? 4 : 16);
+        const CT jac = d == 0 ? a.jx : (d == 1 ? a.jy : a.jz);
+        CT Dd[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) Dd[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], idx);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            CT dd = 0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) dd += Dd[m] * sF[v][base + m * stride];
+            rs[v] += dd * jac;
+        }
+    }
+    CT r[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT val = -rs[v];
+        if (v == 1) val += a.bf;
+        r[v] = a.round_rhs ? (CT)(float)val : val;  // RHS_REAL round trip (rhs.cu:509, rk.cu:52)
+    }
+
+    // Interface fluxes at the 6 own faces -> own-side correction j
+    // (compute_face_j / compute_face_j_wall, rhs.cu:273-339).
+    for (int task = pt; task < 96; task += 64) {
+        const int f = task >> 4, fp = task & 15, d = f >> 1;
+        const bool right = f & 1;
+        int base, stride;
+        face_line(d, fp, base, stride);
+        const int order = face_order(d);
+        CT c0 = right ? o.eR[0] : o.eL[0], c1 = right ? o.eR[1] : o.eL[1];
+        CT c2 = right ? o.eR[2] : o.eL[2], c3 = right ? o.eR[3] : o.eL[3];
+        CT own[5], oth[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) own[v] = trace4(sU[v], base, stride, c0, c1, c2, c3, order);
+        const int64_t nb = f == 0 ? el.nb[0]
+                           : f == 1 ? el.nb[1]
+                           : f == 2 ? el.nb[2]
+                           : f == 3 ? el.nb[3]
+                           : f == 4 ? el.nb[4] : el.nb[5];
+        const bool wall = nb < 0;
+        // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
+        const int rf = d == 0 ? (right ? 6 : 7) : (d == 1 ? (right ? 2 : 3) : (right ? 4 : 5));
+#pragma unroll
+        for (int v = 0; v < 5; ++v) oth[v] = wall ? own[v] : a.tr_in[nb * kTraceStride + rf * 80 + v * 16 + fp];
+        CT fvo[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0}, fvn[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
+        if (VISC) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                fvo[1 + c] = a.fv[e * kFvStride + f * 64 + c * 16 + fp];
+                fvn[1 + c] = wall ? fvo[1 + c] : a.fv[nb * kFvStride + (f ^ 1) * 64 + c * 16 + fp];
+            }
+        }
+        CT L[5], R[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            L[v] = right ? own[v] : oth[v];
+            R[v] = right ? oth[v] : own[v];
+        }
+        CT nrm[3] = {d == 0 ? (CT)1.0 : (CT)0.0, d == 1 ? (CT)1.0 : (CT)0.0, d == 2 ? (CT)1.0 : (CT)0.0};
+        CT fr[5], fe[5];
+        interface_flux(a.flux, L, R, nrm, fr);
+        euler_flux(d, own, fe);
+        CT wall_f[5];
+        {
+            CT v2r = own[2] * own[2] / own[0];  // rhs.cu:323-326
+            wall_f[0] = (CT)0.0;
+            wall_f[1] = (CT)0.0;
+            wall_f[2] = fe[2] - v2r;
+            wall_f[3] = (CT)0.0;
+            wall_f[4] = (CT)0.0;
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            CT jv;
+            if (VISC) {
+                const CT fl = right ? fvo[v] : fvn[v];
+                const CT frr = right ? fvn[v] : fvo[v];
+                CT fc = fr[v] - ((CT)0.5) * (fl + frr);
+                CT jn = fc - (fe[v] - fvo[v]);
+                CT jw = (wall_f[v] - fvo[v]) - (fe[v] - fvo[v]);
+                jv = wall ? jw : jn;
+            } else {
+                jv = wall ? (wall_f[v] - fe[v]) : (fr[v] - fe[v]);
+            }
+            sJ[f][v][fp] = jv;
+        }
+    }
+    __syncthreads();
+
+    // FR correction and stage update (rk.cu:12-25, 63-82).
+    const CT gLi = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], i), gRi = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], i);
+    const CT gLj = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], j), gRj = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], j);
+    const CT gLk = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], k), gRk = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], k);
+    const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
+    UT unew[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT cx = -(sJ[0][v][fpx] * gLi * a.jx) - (sJ[1][v][fpx] * gRi * a.jx);
+        CT cy = -(sJ[2][v][fpy] * gLj * a.jy) - (sJ[3][v][fpy] * gRj * a.jy);
+        CT cz = -(sJ[4][v][fpz] * gLk * a.jz) - (sJ[5][v][fpz] * gRk * a.jz);
+        CT rt = r[v] + (cx + cy + cz);
+        if (a.mode == 1) {
+            a.dbg[v * NP + e * 64 + pt] = rt;
+            continue;
+        }
+        if (!a.accum_state) {
+            CT up = (CT)uprv[v], uc = (CT)ucur[v];
+            CT val = a.stage1 ? (a.ca * up + a.cc * rt) : (a.ca * up + a.cb * uc + a.cc * rt);
+            unew[v] = (UT)val;
+        } else {
+            UT rr = (UT)rt;
+            unew[v] = a.stage1 ? (UT)(a.ua * uprv[v] + a.uc * rr) : (UT)(a.ua * uprv[v] + a.ub * ucur[v] + a.uc * rr);
+        }
+        a.uout[v * NP + e * 64 + pt] = unew[v];
+    }
+    if (a.mode == 1) return;
+
+    // Face traces of the new state for the next stage.
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 5; ++v) sU[v][pt] = (CT)unew[v];
+    __syncthreads();
+    write_traces(o, sU, a.tr_out, e, pt, VISC);
+}
+
+// Conservation sums: per-element GL-weighted sums (sequential over points),
+// then one block reduces them in a fixed order -> deterministic.
+struct W3 {
+    double w[64];
+};
+
+template <class UT>
+__global__ __launch_bounds__(256) void od_sums_partial_kernel(const UT* u, double* partial, int64_t E, W3 w3) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int64_t NP = E * 64;
+    for (int v = 0; v < 5; ++v) {
+        double s = 0.0;
+        for (int pt = 0; pt < 64; ++pt) s += w3.w[pt] * (double)u[v * NP + e * 64 + pt];
+        partial[v * E + e] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void od_sums_final_kernel(const double* partial, double* out, int64_t E, double jac) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    for (int v = 0; v < 5; ++v) {
+        double s = 0.0;
+        for (int64_t e = t; e < E; e += 256) s += partial[v * E + e];
+        red[t] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) red[t] = red[t] + red[t + w];
+            __syncthreads();
+        }
+        if (t == 0) out[v] = red[0] * jac;
+        __syncthreads();
+    }
+}
+
+inline int check_launch() { return hipGetLastError() == hipSuccess ? 0 : 1; }
+
+}  // namespace
+
+template <class UT, class CT>
+int launch_trace(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
+    hipLaunchKernelGGL((od_trace_kernel<UT, CT>), dim3((unsigned)n_elem), dim3(64), 0, (hipStream_t)stream, a);
+    return check_launch();
+}
+
+template <class UT, class CT>
+int launch_visc(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
+    hipLaunchKernelGGL((od_visc_kernel<UT, CT>), dim3((unsigned)n_elem), dim3(64), 0, (hipStream_t)stream, a);
+    return check_launch();
+}
+
+template <class UT, class CT, bool VISC>
+int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
+    hipLaunchKernelGGL((od_update_kernel<UT, CT, VISC>), dim3((unsigned)n_elem), dim3(64), 0, (hipStream_t)stream, a);
+    return check_launch();
+}
+
+template <class UT>
+int launch_sums(const UT* u, double* partial, double* out, int64_t n_elem, const double w3[64], double jac,
+                void* stream) {
+    W3 w;
+    for (int q = 0; q < 64; ++q) w.w[q] = w3[q];
+    unsigned blocks = (unsigned)((n_elem + 255) / 256);
+    hipLaunchKernelGGL((od_sums_partial_kernel<UT>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, u, partial,
+                       n_elem, w);
+    if (check_launch()) return 1;
+    hipLaunchKernelGGL(od_sums_final_kernel, dim3(1), dim3(256), 0, (hipStream_t)stream, partial, out, n_elem, jac);
+    return check_launch();
+}
+
+const char* kernel_name(int kind) {
+    switch (kind) {
+        case 0: return "od_visc_kernel";
+        case 1: return "od_update_kernel";
+        case 2: return "od_trace_kernel";
+        default: return "?";
+    }
+}
+
+#define OD_INST(UT, CT)                                                                  \
+    template int launch_trace<UT, CT>(const StageArgs<UT, CT>&, int64_t, void*);         \
+    template int launch_visc<UT, CT>(const StageArgs<UT, CT>&, int64_t, void*);          \
+    template int launch_update<UT, CT, true>(const StageArgs<UT, CT>&, int64_t, void*);  \
+    template int launch_update<UT, CT, false>(const StageArgs<UT, CT>&, int64_t, void*);
+OD_INST(float, float)
+OD_INST(double, double)
+OD_INST(double, float)
+OD_INST(float, double)
+#undef OD_INST
+template int launch_sums<float>(const float*, double*, double*, int64_t, const double*, double, void*);
+template int launch_sums<double>(const double*, double*, double*, int64_t, const double*, double, void*);
+
+}  // namespace od
