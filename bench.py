@@ -1,0 +1,157 @@
+"""Benchmark: FR P=3 compressible Navier-Stokes RK3 step on one MI355X.
+
+Workload (the headline configuration): 48x48x48 hex elements (7,077,888
+solution points), box 2pi x 2 x pi, slip walls in y, periodic x/z, f32
+state/RHS/flux, SLAU2, Sutherland viscosity, Mach 0.1, Re_bulk 3000
+(mu_ref = rho U_b h / Re = 0.1 / 3000, h = ly/2), laminar channel
+u = 1.5 M (1 - eta^2) plus the wall-damped divergence-free trip (16 modes,
+amplitude 0.05, seed 1), dt from CFL 0.3.
+
+    python bench.py [--gpus 1] [--steps 300] [--warmup 10] [--cpu-steps 5]
+
+A step is one full SSP-RK3 step (3 stages, one hipGraph launch) over the
+resident state.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GDOF-steps/s, 48³ P=3 f32; speedup of 32³+closure vs plain 64³ at equal error"
+HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+REFERENCE_GDOF = 0.114  # README.md:205-209, GTX 1650 Ti, 48^3 f32 (~62 ms/step)
+
+NX = NY = NZ = 48
+LX, LY, LZ = 2 * np.pi, 2.0, np.pi
+MACH, RE_BULK = 0.1, 3000.0
+MU_REF = 1.0 * MACH * (LY / 2) / RE_BULK
+CFL = 0.3
+# Compulsory state traffic of SSP-RK3 with two registers, f32: stage 1 reads
+# u^n and writes u1 (2 x 20 B), stages 2 and 3 read u^n, u_k and write
+# (3 x 20 B): 160 B per DOF-step, i.e. 160/3 B per DOF per update launch.
+ALG_BYTES_PER_DOF_STEP = 160.0
+FLOP_PER_DOF_STEP = 3.0e3
+
+
+def workload_kwargs(real):
+    return dict(u_real=real, flux="slau2", y_wall=True, viscous=True, viscosity_model=1, mu_ref=MU_REF)
+
+
+def traffic_from_profile():
+    """HBM bytes per od_update_kernel launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if the
+    committed summary exists."""
+    path = os.path.join(ROOT, "profiles", "od_update_kernel_48cubed_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps for the CPU baseline (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design, not a build).", file=sys.stderr)
+        sys.exit(2)
+
+    import opendynamo_amd as od
+    from opendynamo_amd import ic
+
+    u0 = ic.channel_state(NX, NY, NZ, LX, LY, LZ, MACH, perturbation_amplitude=0.05, perturbation_modes=16,
+                          perturbation_seed=1, dtype=np.float32)
+    dof = NX * NY * NZ * 64
+    p = od.params(NX, NY, NZ, LX, LY, LZ, **workload_kwargs(od.OD_F32))
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        dt = s.cfl_dt(CFL)
+        s.step(dt, args.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        ms = s.time_steps(dt, args.steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        kern = s.time_kernels(dt, 3)
+        info = s.info()
+        state_ok = bool(np.isfinite(s.get_state()).all())
+    value = dof * args.steps / (ms / 1e3) / 1e9
+    upd_ms = kern["od_update_kernel"]
+    alg_bytes_launch = ALG_BYTES_PER_DOF_STEP / 3.0 * dof
+    achieved = alg_bytes_launch / (upd_ms / 1e3) / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": "od_update_kernel", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_from_profile(),
+        "algorithmic_bytes_per_launch": alg_bytes_launch, "launch_ms": round(upd_ms, 5),
+        "step_frac_alg_hbm": round(value * 1e9 * ALG_BYTES_PER_DOF_STEP / 1e9 / HBM_PEAK_GBS, 5),
+        "valu_frac_alg_flops": round(value * FLOP_PER_DOF_STEP / 1e3 / FP32_VECTOR_PEAK_TFLOPS, 5),
+    }
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GDOF·steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": round(value / REFERENCE_GDOF, 2), "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": "FR P=3 compressible NS, 48x48x48 hex elements (7,077,888 DOF), box 2pi x 2 x pi, "
+                        "y slip walls, x/z periodic, SLAU2, Sutherland, Mach 0.1, Re_bulk 3000, "
+                        "channel IC + 16-mode trip (amp 0.05, seed 1), dt from CFL 0.3",
+            "elements": [NX, NY, NZ], "dof": dof, "dt": dt, "launches_per_step": info["launches_per_step"],
+            "device_bytes": info["device_bytes"], "baseline_reference": "0.114 GDOF-steps/s, GTX 1650 Ti, README.md:205",
+        },
+        "roofline": roofline,
+        "kernel_ms": {k: round(v, 5) for k, v in kern.items()},
+        "wall_s_timed": round(wall, 4),
+        "state_finite": state_ok,
+    }
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        threads = cpu_threads()
+        po = O.params(NX, NY, NZ, LX, LY, LZ, n_threads=threads, **workload_kwargs(8))
+        u64 = u0.astype(np.float64)
+        t0 = time.perf_counter()
+        ref = O.step(po, u64, dt, args.cpu_steps)
+        cpu_s = time.perf_counter() - t0
+        cpu_value = dof * args.cpu_steps / cpu_s / 1e9
+        out["cpu_baseline"] = {
+            "value": round(cpu_value, 6), "unit": "GDOF·steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} RK3 steps of the same 48^3 workload from the same IC, FP64 op-for-op "
+                      f"oracle, {threads} threads, {cpu_s:.1f} s",
+        }
+        out["speedup_vs_cpu"] = round(value / cpu_value, 1)
+        # per-variable error of the f32 GPU pipeline against the FP64 oracle
+        with od.Solver(p) as s:
+            s.set_state(u0)
+            s.step(dt, args.cpu_steps)
+            g = s.get_state().astype(np.float64).reshape(5, -1)
+        r = ref.reshape(5, -1)
+        names = ["rho", "rhou", "rhov", "rhow", "E"]
+        out["error_vs_fp64_oracle"] = {
+            "steps": args.cpu_steps,
+            "max_rel": {n: float(np.max(np.abs(g[v] - r[v])) / np.max(np.abs(r[v]))) for v, n in enumerate(names)},
+            "l2_rel": {n: float(np.linalg.norm(g[v] - r[v]) / np.linalg.norm(r[v])) for v, n in enumerate(names)},
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

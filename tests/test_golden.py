@@ -1,0 +1,65 @@
+"""Golden fixtures: the oracle (CPU) and the HIP library (GPU) both
+reproduce the committed vectors bit for bit."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SMALL = json.load(open(os.path.join(HERE, "golden_small.json")))
+DATA = np.load(os.path.join(HERE, "golden_small.npz"))
+CONFIG1 = json.load(open(os.path.join(HERE, "golden_config1.json")))
+
+
+def _config1():
+    from golden.make_golden import config1
+    return config1()
+
+
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_oracle_reproduces_small_golden(name):
+    m = SMALL[name]
+    p = O.params(*m["dims"], *m["lengths"], **m["params"])
+    out = O.step(p, DATA[name + "_in"], m["dt"], m["steps"])
+    np.testing.assert_array_equal(out, DATA[name + "_out"])
+
+
+def test_oracle_reproduces_config1_digests():
+    p, u = _config1()
+    assert hashlib.sha256(u.tobytes()).hexdigest() == CONFIG1["sha256_in"]
+    assert O.cfl_dt(p, u, 0.3) == CONFIG1["dt"]
+    out = O.step(p, u, CONFIG1["dt"], 10)
+    assert hashlib.sha256(out.tobytes()).hexdigest() == CONFIG1["sha256_after_10"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(SMALL))
+def test_gpu_reproduces_small_golden(name):
+    import opendynamo_amd as od
+    m = SMALL[name]
+    print("CASE golden", name, flush=True)
+    with od.Solver(od.params(*m["dims"], *m["lengths"], **m["params"])) as s:
+        s.set_state(DATA[name + "_in"])
+        s.step(m["dt"], m["steps"])
+        np.testing.assert_array_equal(s.get_state(), DATA[name + "_out"])
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_config1_digests():
+    import opendynamo_amd as od
+    p, u = _config1()
+    print("CASE golden config1 100 steps", flush=True)
+    gp = od.params(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, u_real=8, flux="rusanov", viscous=False)
+    with od.Solver(gp) as s:
+        s.set_state(u)
+        done = 0
+        for n in (1, 10, 100):
+            s.step(CONFIG1["dt"], n - done)
+            done = n
+            got = s.get_state()
+            assert hashlib.sha256(got.tobytes()).hexdigest() == CONFIG1[f"sha256_after_{n}"]
+            np.testing.assert_allclose(s.sums(), CONFIG1[f"sums_after_{n}"], rtol=1e-13, atol=1e-13)
