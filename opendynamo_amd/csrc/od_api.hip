@@ -303,6 +303,7 @@ od_status od_create(const od_params* p, int device, od_ctx** out) {
     if (!(p->lx > 0 && p->ly > 0 && p->lz > 0)) return fail(OD_ERR_INVALID, "domain extents must be positive");
     const int64_t E = (int64_t)p->nx * p->ny * p->nz;
     if (E > (int64_t)0x7fffffff) return fail(OD_ERR_INVALID, "too many elements for one launch grid");
+    if (p->ny > 65535 || p->nz > 65535) return fail(OD_ERR_INVALID, "ny and nz must be <= 65535 (launch grid)");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(OD_ERR_INVALID, "no such HIP device");

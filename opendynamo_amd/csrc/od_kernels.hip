@@ -233,32 +233,34 @@ __device__ __forceinline__ C sutherland(const C* s, C mu_ref, C t_ref, C s_suth)
 }
 
 // ---------------------------------------------------------------------------
-// Element geometry helpers.
+// Work-group layout: two elements per 128-thread work-group, wave w owns
+// element 2*blockIdx.x + w for the point-wise phases (lane = solution point
+// pt = i + 4j + 16k).  Face-point phases work on both elements at once,
+// direction by direction: a direction has 2 elements x 2 faces x 16 points
+// = 64 tasks, one full wave pass with direction-specialised code.  Wave 0
+// takes x and z, wave 1 takes y (3 full passes per 2 elements; a wave that
+// finishes early waits at the barrier and frees its SIMD for other waves).
 // ---------------------------------------------------------------------------
+constexpr int kEpb = 2;
+
 struct Elem {
-    int64_t e;
-    int ex, ey, ez;
-    int64_t nb[6];  // xL, xR, yL, yR, zL, zR neighbour; -1 = slip wall
+    int64_t xl, xr, yl, yr, zl, zr;  // neighbours across each face; -1 = slip wall
 };
 
-__device__ __forceinline__ Elem make_elem(int64_t e, int nx, int ny, int nz, int wall) {
+// Element (ex, ey, ez) and its face neighbours; coordinates come from the
+// launch grid, so no integer division is needed.
+__device__ __forceinline__ Elem make_elem(int ex, int ey, int ez, int nx, int ny, int nz, int wall) {
     Elem r;
-    r.e = e;
-    r.ex = (int)(e % nx);
-    r.ey = (int)((e / nx) % ny);
-    r.ez = (int)(e / ((int64_t)nx * ny));
-    auto id = [&](int x, int y, int z) { return (int64_t)x + (int64_t)nx * (y + (int64_t)ny * z); };
-    r.nb[0] = id((r.ex - 1 + nx) % nx, r.ey, r.ez);
-    r.nb[1] = id((r.ex + 1) % nx, r.ey, r.ez);
-    if (wall) {
-        r.nb[2] = r.ey > 0 ? id(r.ex, r.ey - 1, r.ez) : -1;
-        r.nb[3] = r.ey < ny - 1 ? id(r.ex, r.ey + 1, r.ez) : -1;
-    } else {
-        r.nb[2] = id(r.ex, (r.ey - 1 + ny) % ny, r.ez);
-        r.nb[3] = id(r.ex, (r.ey + 1) % ny, r.ez);
-    }
-    r.nb[4] = id(r.ex, r.ey, (r.ez - 1 + nz) % nz);
-    r.nb[5] = id(r.ex, r.ey, (r.ez + 1) % nz);
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int64_t e = ex + sy * ey + sz * ez;
+    r.xl = e + (ex == 0 ? nx - 1 : -1);
+    r.xr = e + (ex == nx - 1 ? 1 - nx : 1);
+    const int64_t yl = e + (ey == 0 ? (int64_t)(ny - 1) * sy : -sy);
+    const int64_t yr = e + (ey == ny - 1 ? -(int64_t)(ny - 1) * sy : sy);
+    r.yl = (wall && ey == 0) ? -1 : yl;
+    r.yr = (wall && ey == ny - 1) ? -1 : yr;
+    r.zl = e + (ez == 0 ? (int64_t)(nz - 1) * sz : -sz);
+    r.zr = e + (ez == nz - 1 ? -(int64_t)(nz - 1) * sz : sz);
     return r;
 }
 
@@ -276,30 +278,108 @@ __device__ __forceinline__ void face_line(int d, int fp, int& base, int& stride)
 template <class C>
 __device__ __forceinline__ C trace4(const C* q, int base, int stride, C c0, C c1, C c2, C c3, int order) {
     C t0 = q[base] * c0, t1 = q[base + stride] * c1, t2 = q[base + 2 * stride] * c2, t3 = q[base + 3 * stride] * c3;
-    C pw = (t0 + t1) + (t2 + t3);
-    C s0 = (C)0;
-    s0 += t0;
-    s0 += t1;
-    s0 += t2;
-    s0 += t3;
-    C s1 = t0 + t1 + t2 + t3;
-    return order == 0 ? pw : (order == 1 ? s0 : s1);
+    if (order == 0) return (t0 + t1) + (t2 + t3);
+    if (order == 1) {
+        C s0 = (C)0;
+        s0 += t0;
+        s0 += t1;
+        s0 += t2;
+        s0 += t3;
+        return s0;
+    }
+    return t0 + t1 + t2 + t3;
 }
 
-// Riemann-input / gradient-slice order of the face normal to d (x: seq from
-// first, y: pairwise, z: seq from 0).
-__device__ __forceinline__ int face_order(int d) { return d == 0 ? 2 : (d == 1 ? 0 : 1); }
+// Order of the BR1 traces (rhs.cu:385-455): x, y pairwise; z sequential from 0.
+__device__ __forceinline__ constexpr int br1_order(int d) { return d == 2 ? 1 : 0; }
+// Order of the Riemann inputs and gradient slices (rhs.cu:126-180, 540-543,
+// 663-674, 716-727): x sequential from the first product, y pairwise, z from 0.
+__device__ __forceinline__ constexpr int face_order(int d) { return d == 0 ? 2 : (d == 1 ? 0 : 1); }
 
 template <class C>
-__device__ __forceinline__ void accumulate_jump(C& gc, C own, C nb, C gp, C jac) {  // rhs.cu:105-108
-    C jmp = ((C)0.5) * (nb + own) - own;
-    gc += jmp * gp * jac;
+struct Coef4 {
+    C c0, c1, c2, c3;
+};
+template <class C>
+__device__ __forceinline__ Coef4<C> side_coef(const Ops<C>& o, bool right) {
+    return {right ? o.eR[0] : o.eL[0], right ? o.eR[1] : o.eL[1], right ? o.eR[2] : o.eL[2],
+            right ? o.eR[3] : o.eL[3]};
 }
 
-// BR1-corrected gradient at the lane's point (rhs.cu:366-465), Q round trip
-// included.  sU: the element's state in LDS; neighbour traces from tr_in.
+// Task of a direction-D face pass: lane -> (local element, side, face point).
+struct FaceTask {
+    int le, side, fp, f;
+};
+template <int D>
+__device__ __forceinline__ FaceTask face_task(int lane) {
+    FaceTask t;
+    t.le = lane >> 5;
+    t.side = (lane >> 4) & 1;
+    t.fp = lane & 15;
+    t.f = 2 * D + t.side;
+    return t;
+}
+
+// The work-group's two elements, held in named scalars (no private arrays).
+struct Tile {
+    int64_t e0, e1;
+    bool v0, v1;
+    Elem el0, el1;
+};
+
+// Launch grid: blockIdx = (pair of x elements, ey, ez).
 template <class UT, class CT>
-__device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const CT (*sU)[64], const Elem& el,
+__device__ __forceinline__ Tile make_tile(const StageArgs<UT, CT>& a) {
+    Tile t;
+    const int ex0 = blockIdx.x * kEpb, ey = blockIdx.y, ez = blockIdx.z;
+    const int64_t row = (int64_t)a.nx * (ey + (int64_t)a.ny * ez);
+    t.e0 = row + ex0;
+    t.e1 = t.e0 + 1;
+    t.v0 = true;
+    t.v1 = ex0 + 1 < a.nx;
+    t.el0 = make_elem(ex0, ey, ez, a.nx, a.ny, a.nz, a.wall);
+    t.el1 = make_elem(t.v1 ? ex0 + 1 : ex0, ey, ez, a.nx, a.ny, a.nz, a.wall);
+    return t;
+}
+
+__device__ __forceinline__ bool tile_valid(const Tile& t, int le) { return le ? t.v1 : t.v0; }
+__device__ __forceinline__ int64_t tile_eidx(const Tile& t, int le) { return le ? t.e1 : t.e0; }
+// Neighbour across the side `side` face of direction D of local element le.
+template <int D>
+__device__ __forceinline__ int64_t tile_nb(const Tile& t, int le, int side) {
+    const Elem& a = t.el0;
+    const Elem& b = t.el1;
+    int64_t l0 = D == 0 ? a.xl : (D == 1 ? a.yl : a.zl), r0 = D == 0 ? a.xr : (D == 1 ? a.yr : a.zr);
+    int64_t l1 = D == 0 ? b.xl : (D == 1 ? b.yl : b.zl), r1 = D == 0 ? b.xr : (D == 1 ? b.yr : b.zr);
+    int64_t n0 = side ? r0 : l0, n1 = side ? r1 : l1;
+    return le ? n1 : n0;
+}
+
+// BR1 jump 1/2 (u_nb + u_own) - u_own at every face point of direction D
+// (accumulate_jump, rhs.cu:105-108; wall ghost rhs.cu:111-114), once per face
+// point instead of once per lane of the normal line.
+template <int D, class UT, class CT>
+__device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (*sU)[5][64], CT (*sJmp)[6][5][16],
+                                          const Tile& tl, int lane) {
+    const FaceTask t = face_task<D>(lane);
+    if (!tile_valid(tl, t.le)) return;
+    int base, stride;
+    face_line(D, t.fp, base, stride);
+    const Coef4<CT> c = side_coef(a.ops, t.side);
+    const int64_t nb = tile_nb<D>(tl, t.le, t.side);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        CT own = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, br1_order(D));
+        const CT sign = (v > 0 && v < 4) ? ((CT)-1.0) : ((CT)1.0);
+        CT nbv = nb >= 0 ? a.tr_in[nb * kTraceStride + (t.f ^ 1) * 80 + v * 16 + t.fp] : own * sign;
+        sJmp[t.le][t.f][v][t.fp] = ((CT)0.5) * (nbv + own) - own;
+    }
+}
+
+// BR1-corrected gradient at the lane's point (rhs.cu:366-465) from the
+// element's LDS tile and the face jumps; Q round trip included.
+template <class UT, class CT>
+__device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const CT (*sU)[64], const CT (*sJmp)[5][16],
                                               int i, int j, int k, CT* g) {
     const Ops<CT>& o = a.ops;
     CT Di[4], Dj[4], Dk[4];
@@ -311,7 +391,9 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
     }
     const int xb = 4 * j + 16 * k, yb = i + 16 * k, zb = i + 4 * j;
     const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
-    CT gx[5], gy[5], gz[5];
+    const CT gLi = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], i), gRi = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], i);
+    const CT gLj = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], j), gRj = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], j);
+    const CT gLk = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], k), gRk = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], k);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         CT dz = 0, dxi = 0, deta = 0;
@@ -321,77 +403,144 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
         for (int m = 0; m < 4; ++m) dxi += Di[m] * sU[v][xb + m];
 #pragma unroll
         for (int m = 0; m < 4; ++m) deta += Dj[m] * sU[v][yb + 4 * m];
-        gx[v] = dxi * a.jx;
-        gy[v] = deta * a.jy;
-        gz[v] = dz * a.jz;
-    }
-    const CT gLi = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], i), gRi = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], i);
-    const CT gLj = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], j), gRj = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], j);
-    const CT gLk = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], k), gRk = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], k);
-    const CT* tr = a.tr_in;
-    // x faces: own and neighbour traces pairwise (rhs.cu:385-399)
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        CT own_l = trace4(sU[v], xb, 1, o.eL[0], o.eL[1], o.eL[2], o.eL[3], 0);
-        CT own_r = trace4(sU[v], xb, 1, o.eR[0], o.eR[1], o.eR[2], o.eR[3], 0);
-        CT nb_l = tr[el.nb[0] * kTraceStride + 1 * 80 + v * 16 + fpx];
-        CT nb_r = tr[el.nb[1] * kTraceStride + 0 * 80 + v * 16 + fpx];
-        accumulate_jump(gx[v], own_l, nb_l, gLi, a.jx);
-        accumulate_jump(gx[v], own_r, nb_r, gRi, a.jx);
-    }
-    // y faces, slip-wall ghost negates the momenta (rhs.cu:401-438, 111-114)
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        CT own_l = trace4(sU[v], yb, 4, o.eL[0], o.eL[1], o.eL[2], o.eL[3], 0);
-        CT own_r = trace4(sU[v], yb, 4, o.eR[0], o.eR[1], o.eR[2], o.eR[3], 0);
-        const CT sign = (v > 0 && v < 4) ? ((CT)-1.0) : ((CT)1.0);
-        CT nb_l = el.nb[2] >= 0 ? tr[el.nb[2] * kTraceStride + 3 * 80 + v * 16 + fpy] : own_l * sign;
-        CT nb_r = el.nb[3] >= 0 ? tr[el.nb[3] * kTraceStride + 2 * 80 + v * 16 + fpy] : own_r * sign;
-        accumulate_jump(gy[v], own_l, nb_l, gLj, a.jy);
-        accumulate_jump(gy[v], own_r, nb_r, gRj, a.jy);
-    }
-    // z faces: sequential sums from 0 (rhs.cu:440-455)
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        CT own_l = trace4(sU[v], zb, 16, o.eL[0], o.eL[1], o.eL[2], o.eL[3], 1);
-        CT own_r = trace4(sU[v], zb, 16, o.eR[0], o.eR[1], o.eR[2], o.eR[3], 1);
-        CT nb_l = tr[el.nb[4] * kTraceStride + 5 * 80 + v * 16 + fpz];
-        CT nb_r = tr[el.nb[5] * kTraceStride + 4 * 80 + v * 16 + fpz];
-        accumulate_jump(gz[v], own_l, nb_l, gLk, a.jz);
-        accumulate_jump(gz[v], own_r, nb_r, gRk, a.jz);
-    }
-    // Q_REAL round trip (rhs.cu:457-461): only narrows when CT is double.
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        CT x = gx[v], y = gy[v], z = gz[v];
-        if (a.round_q) {
-            x = (CT)(float)x;
-            y = (CT)(float)y;
-            z = (CT)(float)z;
+        CT gx = dxi * a.jx, gy = deta * a.jy, gz = dz * a.jz;
+        gx += sJmp[0][v][fpx] * gLi * a.jx;
+        gx += sJmp[1][v][fpx] * gRi * a.jx;
+        gy += sJmp[2][v][fpy] * gLj * a.jy;
+        gy += sJmp[3][v][fpy] * gRj * a.jy;
+        gz += sJmp[4][v][fpz] * gLk * a.jz;
+        gz += sJmp[5][v][fpz] * gRk * a.jz;
+        if (a.round_q) {  // Q_REAL round trip (rhs.cu:457-461), narrows only when CT is double
+            gx = (CT)(float)gx;
+            gy = (CT)(float)gy;
+            gz = (CT)(float)gz;
         }
-        g[3 * v + 0] = x;
-        g[3 * v + 1] = y;
-        g[3 * v + 2] = z;
+        g[3 * v + 0] = gx;
+        g[3 * v + 1] = gy;
+        g[3 * v + 2] = gz;
     }
 }
 
-// Face traces of the state in sU -> tr_out (layout in od_internal.hpp).
-template <class CT>
-__device__ __forceinline__ void write_traces(const Ops<CT>& o, const CT (*sU)[64], CT* tr_out, int64_t e, int lane,
-                                             bool need_pairwise_x) {
-    for (int task = lane; task < 128; task += 64) {
-        int f = task >> 4, fp = task & 15;
-        if (!need_pairwise_x && f < 2) continue;
-        int d = f < 6 ? (f >> 1) : 0;
-        int order = f < 4 ? 0 : (f < 6 ? 1 : 2);
-        int base, stride;
-        face_line(d, fp, base, stride);
-        bool right = f & 1;
-        CT c0 = right ? o.eR[0] : o.eL[0], c1 = right ? o.eR[1] : o.eL[1];
-        CT c2 = right ? o.eR[2] : o.eL[2], c3 = right ? o.eR[3] : o.eL[3];
-        CT* dst = tr_out + e * kTraceStride + f * 80 + fp;
+template <class UT, class CT>
+__device__ __forceinline__ void run_br1_jumps(const StageArgs<UT, CT>& a, const CT (*sU)[5][64],
+                                              CT (*sJmp)[6][5][16], const Tile& tl, int wave, int lane) {
+    if (wave == 0) {
+        br1_jumps<0>(a, sU, sJmp, tl, lane);
+        br1_jumps<2>(a, sU, sJmp, tl, lane);
+    } else {
+        br1_jumps<1>(a, sU, sJmp, tl, lane);
+    }
+}
+
+// Viscous normal flux at the element faces of direction D (the
+// dispatch_viscous_flux calls of compute_face_j / compute_face_j_wall,
+// rhs.cu:284-285, 318) from the Riemann-order state trace and the gradient
+// slice; only the 11 gradient components the direction reads are traced.
+template <int D, class UT, class CT>
+__device__ __forceinline__ void fv_pass(const StageArgs<UT, CT>& a, const CT (*sU)[5][64], const CT (*sG)[15][64],
+                                        const Tile& tl, int lane) {
+    const FaceTask t = face_task<D>(lane);
+    if (!tile_valid(tl, t.le)) return;
+    int base, stride;
+    face_line(D, t.fp, base, stride);
+    const Coef4<CT> c = side_coef(a.ops, t.side);
+    CT st[5], gt[15], fvv[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) dst[v * 16] = trace4(sU[v], base, stride, c0, c1, c2, c3, order);
+    for (int v = 0; v < 5; ++v) st[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+#pragma unroll
+    for (int q = 0; q < 15; ++q) gt[q] = trace4(sG[t.le][q], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+    CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+    viscous_flux(D, st, gt, mu, fvv);
+    CT* dst = a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q * 16] = fvv[1 + q];
+}
+
+// Interface flux at the element faces of direction D -> own-side correction
+// j into LDS (compute_face_j / compute_face_j_wall, rhs.cu:273-339).
+template <int D, bool VISC, class UT, class CT>
+__device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (*sU)[5][64], CT (*sJ)[6][5][16],
+                                          const Tile& tl, int lane) {
+    const FaceTask t = face_task<D>(lane);
+    if (!tile_valid(tl, t.le)) return;
+    int base, stride;
+    face_line(D, t.fp, base, stride);
+    const Coef4<CT> c = side_coef(a.ops, t.side);
+    const bool right = t.side;
+    CT own[5], oth[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) own[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+    const int64_t nb = tile_nb<D>(tl, t.le, t.side);
+    const bool wall = nb < 0;
+    // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
+    const int rf = D == 0 ? (right ? 6 : 7) : (D == 1 ? (right ? 2 : 3) : (right ? 4 : 5));
+#pragma unroll
+    for (int v = 0; v < 5; ++v) oth[v] = wall ? own[v] : a.tr_in[nb * kTraceStride + rf * 80 + v * 16 + t.fp];
+    CT fvo[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0}, fvn[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
+    if (VISC) {
+        const CT* src_o = a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) fvo[1 + q] = src_o[q * 16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) fvn[1 + q] = wall ? fvo[1 + q] : a.fv[nb * kFvStride + (t.f ^ 1) * 64 + q * 16 + t.fp];
+    }
+    CT L[5], R[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        L[v] = right ? own[v] : oth[v];
+        R[v] = right ? oth[v] : own[v];
+    }
+    const CT nrm[3] = {D == 0 ? (CT)1.0 : (CT)0.0, D == 1 ? (CT)1.0 : (CT)0.0, D == 2 ? (CT)1.0 : (CT)0.0};
+    CT fr[5], fe[5];
+    interface_flux(a.flux, L, R, nrm, fr);
+    euler_flux(D, own, fe);
+    CT wall_p = (CT)0;
+    if (D == 1) wall_p = fe[2] - own[2] * own[2] / own[0];  // rhs.cu:323-326
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const CT wf = (D == 1 && v == 2) ? wall_p : (CT)0.0;
+        CT jv;
+        if (VISC) {
+            const CT fl = right ? fvo[v] : fvn[v];
+            const CT frr = right ? fvn[v] : fvo[v];
+            CT fc = fr[v] - ((CT)0.5) * (fl + frr);
+            CT jn = fc - (fe[v] - fvo[v]);
+            CT jw = (wf - fvo[v]) - (fe[v] - fvo[v]);
+            jv = wall ? jw : jn;
+        } else {
+            jv = wall ? (wf - fe[v]) : (fr[v] - fe[v]);
+        }
+        sJ[t.le][t.f][v][t.fp] = jv;
+    }
+}
+
+// Face traces of the state tile -> tr_out.  Wave 0 writes the x traces
+// (pairwise and sequential orders), wave 1 the y and z traces; each wave
+// covers both elements in two passes.
+template <class CT>
+__device__ __forceinline__ void write_traces2(const Ops<CT>& o, const CT (*sU)[5][64], CT* tr_out, const Tile& tl,
+                                              int wave, int lane, bool need_pairwise_x) {
+    const int fsel = lane >> 4, fp = lane & 15;
+    int f, d, order;
+    if (wave == 0) {
+        f = fsel < 2 ? fsel : fsel + 4;  // 0, 1 pairwise; 6, 7 sequential from the first product
+        d = 0;
+        order = fsel < 2 ? 0 : 2;
+    } else {
+        f = 2 + fsel;  // 2, 3 y pairwise; 4, 5 z sequential from 0
+        d = fsel < 2 ? 1 : 2;
+        order = fsel < 2 ? 0 : 1;
+    }
+    if (wave == 0 && !need_pairwise_x && fsel < 2) return;
+    int base, stride;
+    face_line(d, fp, base, stride);
+    const Coef4<CT> c = side_coef(o, f & 1);
+#pragma unroll
+    for (int le = 0; le < kEpb; ++le) {
+        if (!tile_valid(tl, le)) continue;
+        CT* dst = tr_out + tile_eidx(tl, le) * kTraceStride + f * 80 + fp;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) dst[v * 16] = trace4(sU[le][v], base, stride, c.c0, c.c1, c.c2, c.c3, order);
     }
 }
 
@@ -399,86 +548,82 @@ __device__ __forceinline__ void write_traces(const Ops<CT>& o, const CT (*sU)[64
 // Kernels.
 // ---------------------------------------------------------------------------
 template <class UT, class CT>
-__global__ __launch_bounds__(64) void od_trace_kernel(StageArgs<UT, CT> a) {
-    __shared__ CT sU[5][64];
-    const int pt = threadIdx.x;
-    const int64_t e = blockIdx.x;
+__global__ __launch_bounds__(128) void od_trace_kernel(StageArgs<UT, CT> a) {
+    __shared__ CT sU[kEpb][5][64];
+    const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
+    const Tile tl = make_tile(a);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    if (tile_valid(tl, wave)) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) sU[v][pt] = (CT)a.uin[v * NP + e * 64 + pt];
+        for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
+    }
     __syncthreads();
-    write_traces(a.ops, sU, a.tr_out, e, pt, true);
+    write_traces2(a.ops, sU, a.tr_out, tl, wave, pt, true);
 }
 
 template <class UT, class CT>
-__global__ __launch_bounds__(64) void od_visc_kernel(StageArgs<UT, CT> a) {
-    __shared__ CT sU[5][64];
-    __shared__ CT sG[15][64];
-    const int pt = threadIdx.x;
-    const int64_t e = blockIdx.x;
-    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+__global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
+    __shared__ CT sU[kEpb][5][64];
+    __shared__ CT sG[kEpb][15][64];
+    __shared__ CT sJmp[kEpb][6][5][16];
+    const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Elem el = make_elem(e, a.nx, a.ny, a.nz, a.wall);
+    const Tile tl = make_tile(a);
+    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    if (tile_valid(tl, wave)) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) sU[v][pt] = (CT)a.uin[v * NP + e * 64 + pt];
+        for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
+    }
     __syncthreads();
-    CT g[15];
-    lane_gradient(a, sU, el, i, j, k, g);
-#pragma unroll
-    for (int c = 0; c < 15; ++c) sG[c][pt] = g[c];
+    run_br1_jumps(a, sU, sJmp, tl, wave, pt);
     __syncthreads();
-    const Ops<CT>& o = a.ops;
-    // viscous normal flux at the 6 own faces, 16 points each (the
-    // dispatch_viscous_flux calls of compute_face_j / compute_face_j_wall,
-    // rhs.cu:284-285, 318), from the Riemann-order state trace and the
-    // gradient slice (rhs.cu:126-180, 540-543, 663-674, 716-727).
-    for (int task = pt; task < 96; task += 64) {
-        int f = task >> 4, fp = task & 15, d = f >> 1;
-        int base, stride;
-        face_line(d, fp, base, stride);
-        int order = face_order(d);
-        bool right = f & 1;
-        CT c0 = right ? o.eR[0] : o.eL[0], c1 = right ? o.eR[1] : o.eL[1];
-        CT c2 = right ? o.eR[2] : o.eL[2], c3 = right ? o.eR[3] : o.eL[3];
-        CT st[5], gt[15], fvv[5];
+    if (tile_valid(tl, wave)) {
+        CT g[15];
+        lane_gradient(a, sU[wave], sJmp[wave], i, j, k, g);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) st[v] = trace4(sU[v], base, stride, c0, c1, c2, c3, order);
-#pragma unroll
-        for (int c = 0; c < 15; ++c) gt[c] = trace4(sG[c], base, stride, c0, c1, c2, c3, order);
-        CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
-        viscous_flux(d, st, gt, mu, fvv);
-        CT* dst = a.fv + e * kFvStride + f * 64 + fp;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) dst[c * 16] = fvv[1 + c];
+        for (int q = 0; q < 15; ++q) sG[wave][q][pt] = g[q];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        fv_pass<0>(a, sU, sG, tl, pt);
+        fv_pass<2>(a, sU, sG, tl, pt);
+    } else {
+        fv_pass<1>(a, sU, sG, tl, pt);
     }
 }
 
 template <class UT, class CT, bool VISC>
-__global__ __launch_bounds__(64) void od_update_kernel(StageArgs<UT, CT> a) {
-    __shared__ CT sU[5][64];
-    __shared__ CT sF[5][64];
-    __shared__ CT sJ[6][5][16];
-    const int pt = threadIdx.x;
-    const int64_t e = blockIdx.x;
-    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+__global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
+    __shared__ CT sU[kEpb][5][64];
+    __shared__ CT sF[kEpb][5][64];
+    __shared__ CT sJ[kEpb][6][5][16];
+    __shared__ CT sJmp[kEpb][6][5][16];
+    const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Elem el = make_elem(e, a.nx, a.ny, a.nz, a.wall);
+    const Tile tl = make_tile(a);
+    const bool mine = tile_valid(tl, wave);
+    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    const int64_t gbase = tile_eidx(tl, wave) * 64 + pt;
     const Ops<CT>& o = a.ops;
 
     UT ucur[5], uprv[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        ucur[v] = a.uin[v * NP + e * 64 + pt];
-        uprv[v] = a.uprev[v * NP + e * 64 + pt];
-        sU[v][pt] = (CT)ucur[v];
+        ucur[v] = mine ? a.uin[v * NP + gbase] : (UT)1;
+        uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
+        sU[wave][v][pt] = (CT)ucur[v];
     }
     __syncthreads();
 
     CT g[15];
-    if (VISC) lane_gradient(a, sU, el, i, j, k, g);
+    if (VISC) {
+        run_br1_jumps(a, sU, sJmp, tl, wave, pt);
+        __syncthreads();
+        lane_gradient(a, sU[wave], sJmp[wave], i, j, k, g);
+    }
     CT st[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) st[v] = sU[v][pt];
+    for (int v = 0; v < 5; ++v) st[v] = sU[wave][v][pt];
     CT mu = (CT)0;
     if (VISC) mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
 
@@ -499,7 +644,7 @@ __global__ __launch_bounds__(64) void od_update_kernel(StageArgs<UT, CT> a) {
         }
         if (d > 0) __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 5; ++v) sF[v][pt] = F[v];
+        for (int v = 0; v < 5; ++v) sF[wave][v][pt] = F[v];
         __syncthreads();
         const int idx = d == 0 ? i : (d == 1 ? j : k);
         const int base = d == 0 ? 4 * j + 16 * k : (d == 1 ? i + 16 * k : i + 4 * j);
@@ -512,7 +657,7 @@ __global__ __launch_bounds__(64) void od_update_kernel(StageArgs<UT, CT> a) {
         for (int v = 0; v < 5; ++v) {
             CT dd = 0;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) dd += Dd[m] * sF[v][base + m * stride];
+            for (int m = 0; m < 4; ++m) dd += Dd[m] * sF[wave][v][base + m * stride];
             rs[v] += dd * jac;
         }
     }
@@ -524,71 +669,12 @@ __global__ __launch_bounds__(64) void od_update_kernel(StageArgs<UT, CT> a) {
         r[v] = a.round_rhs ? (CT)(float)val : val;  // RHS_REAL round trip (rhs.cu:509, rk.cu:52)
     }
 
-    // Interface fluxes at the 6 own faces -> own-side correction j
-    // (compute_face_j / compute_face_j_wall, rhs.cu:273-339).
-    for (int task = pt; task < 96; task += 64) {
-        const int f = task >> 4, fp = task & 15, d = f >> 1;
-        const bool right = f & 1;
-        int base, stride;
-        face_line(d, fp, base, stride);
-        const int order = face_order(d);
-        CT c0 = right ? o.eR[0] : o.eL[0], c1 = right ? o.eR[1] : o.eL[1];
-        CT c2 = right ? o.eR[2] : o.eL[2], c3 = right ? o.eR[3] : o.eL[3];
-        CT own[5], oth[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) own[v] = trace4(sU[v], base, stride, c0, c1, c2, c3, order);
-        const int64_t nb = f == 0 ? el.nb[0]
-                           : f == 1 ? el.nb[1]
-                           : f == 2 ? el.nb[2]
-                           : f == 3 ? el.nb[3]
-                           : f == 4 ? el.nb[4] : el.nb[5];
-        const bool wall = nb < 0;
-        // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
-        const int rf = d == 0 ? (right ? 6 : 7) : (d == 1 ? (right ? 2 : 3) : (right ? 4 : 5));
-#pragma unroll
-        for (int v = 0; v < 5; ++v) oth[v] = wall ? own[v] : a.tr_in[nb * kTraceStride + rf * 80 + v * 16 + fp];
-        CT fvo[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0}, fvn[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
-        if (VISC) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                fvo[1 + c] = a.fv[e * kFvStride + f * 64 + c * 16 + fp];
-                fvn[1 + c] = wall ? fvo[1 + c] : a.fv[nb * kFvStride + (f ^ 1) * 64 + c * 16 + fp];
-            }
-        }
-        CT L[5], R[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            L[v] = right ? own[v] : oth[v];
-            R[v] = right ? oth[v] : own[v];
-        }
-        CT nrm[3] = {d == 0 ? (CT)1.0 : (CT)0.0, d == 1 ? (CT)1.0 : (CT)0.0, d == 2 ? (CT)1.0 : (CT)0.0};
-        CT fr[5], fe[5];
-        interface_flux(a.flux, L, R, nrm, fr);
-        euler_flux(d, own, fe);
-        CT wall_f[5];
-        {
-            CT v2r = own[2] * own[2] / own[0];  // rhs.cu:323-326
-            wall_f[0] = (CT)0.0;
-            wall_f[1] = (CT)0.0;
-            wall_f[2] = fe[2] - v2r;
-            wall_f[3] = (CT)0.0;
-            wall_f[4] = (CT)0.0;
-        }
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            CT jv;
-            if (VISC) {
-                const CT fl = right ? fvo[v] : fvn[v];
-                const CT frr = right ? fvn[v] : fvo[v];
-                CT fc = fr[v] - ((CT)0.5) * (fl + frr);
-                CT jn = fc - (fe[v] - fvo[v]);
-                CT jw = (wall_f[v] - fvo[v]) - (fe[v] - fvo[v]);
-                jv = wall ? jw : jn;
-            } else {
-                jv = wall ? (wall_f[v] - fe[v]) : (fr[v] - fe[v]);
-            }
-            sJ[f][v][fp] = jv;
-        }
+    // Interface fluxes at all own faces, direction by direction.
+    if (wave == 0) {
+        flux_pass<0, VISC>(a, sU, sJ, tl, pt);
+        flux_pass<2, VISC>(a, sU, sJ, tl, pt);
+    } else {
+        flux_pass<1, VISC>(a, sU, sJ, tl, pt);
     }
     __syncthreads();
 
@@ -600,12 +686,14 @@ __global__ __launch_bounds__(64) void od_update_kernel(StageArgs<UT, CT> a) {
     UT unew[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        CT cx = -(sJ[0][v][fpx] * gLi * a.jx) - (sJ[1][v][fpx] * gRi * a.jx);
-        CT cy = -(sJ[2][v][fpy] * gLj * a.jy) - (sJ[3][v][fpy] * gRj * a.jy);
-        CT cz = -(sJ[4][v][fpz] * gLk * a.jz) - (sJ[5][v][fpz] * gRk * a.jz);
+        const CT(*J)[5][16] = sJ[wave];
+        CT cx = -(J[0][v][fpx] * gLi * a.jx) - (J[1][v][fpx] * gRi * a.jx);
+        CT cy = -(J[2][v][fpy] * gLj * a.jy) - (J[3][v][fpy] * gRj * a.jy);
+        CT cz = -(J[4][v][fpz] * gLk * a.jz) - (J[5][v][fpz] * gRk * a.jz);
         CT rt = r[v] + (cx + cy + cz);
         if (a.mode == 1) {
-            a.dbg[v * NP + e * 64 + pt] = rt;
+            if (mine) a.dbg[v * NP + gbase] = rt;
+            unew[v] = ucur[v];
             continue;
         }
         if (!a.accum_state) {
@@ -616,16 +704,16 @@ __global__ __launch_bounds__(64) void od_update_kernel(StageArgs<UT, CT> a) {
             UT rr = (UT)rt;
             unew[v] = a.stage1 ? (UT)(a.ua * uprv[v] + a.uc * rr) : (UT)(a.ua * uprv[v] + a.ub * ucur[v] + a.uc * rr);
         }
-        a.uout[v * NP + e * 64 + pt] = unew[v];
+        if (mine) a.uout[v * NP + gbase] = unew[v];
     }
     if (a.mode == 1) return;
 
     // Face traces of the new state for the next stage.
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < 5; ++v) sU[v][pt] = (CT)unew[v];
+    for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)unew[v];
     __syncthreads();
-    write_traces(o, sU, a.tr_out, e, pt, VISC);
+    write_traces2(o, sU, a.tr_out, tl, wave, pt, VISC);
 }
 
 // Conservation sums: per-element GL-weighted sums (sequential over points),
@@ -665,23 +753,28 @@ __global__ __launch_bounds__(256) void od_sums_final_kernel(const double* partia
 
 inline int check_launch() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 
+template <class UT, class CT>
+dim3 grid_of(const StageArgs<UT, CT>& a) {
+    return dim3((unsigned)((a.nx + kEpb - 1) / kEpb), (unsigned)a.ny, (unsigned)a.nz);
+}
+
 }  // namespace
 
 template <class UT, class CT>
 int launch_trace(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
-    hipLaunchKernelGGL((od_trace_kernel<UT, CT>), dim3((unsigned)n_elem), dim3(64), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL((od_trace_kernel<UT, CT>), grid_of(a), dim3(64 * kEpb), 0, (hipStream_t)stream, a);
     return check_launch();
 }
 
 template <class UT, class CT>
 int launch_visc(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
-    hipLaunchKernelGGL((od_visc_kernel<UT, CT>), dim3((unsigned)n_elem), dim3(64), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL((od_visc_kernel<UT, CT>), grid_of(a), dim3(64 * kEpb), 0, (hipStream_t)stream, a);
     return check_launch();
 }
 
 template <class UT, class CT, bool VISC>
 int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
-    hipLaunchKernelGGL((od_update_kernel<UT, CT, VISC>), dim3((unsigned)n_elem), dim3(64), 0, (hipStream_t)stream, a);
+    hipLaunchKernelGGL((od_update_kernel<UT, CT, VISC>), grid_of(a), dim3(64 * kEpb), 0, (hipStream_t)stream, a);
     return check_launch();
 }
 
