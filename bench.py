@@ -7,7 +7,7 @@ state/RHS/flux, SLAU2, Sutherland viscosity, Mach 0.1, Re_bulk 3000
 u = 1.5 M (1 - eta^2) plus the wall-damped divergence-free trip (16 modes,
 amplitude 0.05, seed 1), dt from CFL 0.3.
 
-    python bench.py [--gpus 1] [--steps 300] [--warmup 10] [--cpu-steps 5]
+    python bench.py [--gpus 1] [--steps 300] [--warmup 10] [--cpu-steps 20] [--copy-calibration]
 
 A step is one full SSP-RK3 step (3 stages, one hipGraph launch) over the
 resident state.  Prints one JSON line.
@@ -56,6 +56,25 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+def hip_copy_calibration(nbytes):
+    """Three device-to-device copies of `nbytes` through the HIP runtime (its
+    blit copy kernel): the known-size reference for FETCH_SIZE / WRITE_SIZE."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("hipMalloc failed in the copy calibration")
+    hip.hipMemset(a, 0, n)
+    for _ in range(3):
+        hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"kernel": "HIP runtime device-to-device copy", "bytes_read": nbytes, "bytes_written": nbytes,
+            "launches": 3}
+
+
 def cpu_threads():
     try:
         n = len(os.sched_getaffinity(0))
@@ -69,7 +88,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=20, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy of known size, the reference "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -150,6 +172,8 @@ def main():
             "max_rel": {n: float(np.max(np.abs(g[v] - r[v])) / np.max(np.abs(r[v]))) for v, n in enumerate(names)},
             "l2_rel": {n: float(np.linalg.norm(g[v] - r[v]) / np.linalg.norm(r[v])) for v, n in enumerate(names)},
         }
+    if args.copy_calibration:
+        out["copy_calibration"] = hip_copy_calibration(1 << 30)
     print(json.dumps(out), flush=True)
 
 
