@@ -41,8 +41,42 @@ ALG_BYTES_PER_DOF_STEP = 160.0
 FLOP_PER_DOF_STEP = 3.0e3
 
 
-def workload_kwargs(real):
-    return dict(u_real=real, flux="slau2", y_wall=True, viscous=True, viscosity_model=1, mu_ref=MU_REF)
+def workload_kwargs(real, fast_math=0):
+    kw = dict(u_real=real, flux="slau2", y_wall=True, viscous=True, viscosity_model=1, mu_ref=MU_REF)
+    if fast_math:
+        kw["fast_math"] = 1
+    return kw
+
+
+def run_gpu(od, u0, p, steps, warmup):
+    """Graph-timed steps + per-kernel event timing on the resident state."""
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        dt = s.cfl_dt(CFL)
+        s.step(dt, warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        ms = s.time_steps(dt, steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        kern = s.time_kernels(dt, 3)
+        info = s.info()
+        ok = bool(np.isfinite(s.get_state()).all())
+    return dict(dt=dt, ms=ms, wall=wall, kern=kern, info=info, ok=ok)
+
+
+def error_vs(ref, od, u0, p, dt, steps):
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        s.step(dt, steps)
+        g = s.get_state().astype(np.float64).reshape(5, -1)
+    r = ref.reshape(5, -1)
+    names = ["rho", "rhou", "rhov", "rhow", "E"]
+    return {
+        "steps": steps,
+        "max_rel": {n: float(np.max(np.abs(g[v] - r[v])) / np.max(np.abs(r[v]))) for v, n in enumerate(names)},
+        "l2_rel": {n: float(np.linalg.norm(g[v] - r[v]) / np.linalg.norm(r[v])) for v, n in enumerate(names)},
+    }
 
 
 def traffic_from_profile():
@@ -104,19 +138,13 @@ def main():
     u0 = ic.channel_state(NX, NY, NZ, LX, LY, LZ, MACH, perturbation_amplitude=0.05, perturbation_modes=16,
                           perturbation_seed=1, dtype=np.float32)
     dof = NX * NY * NZ * 64
-    p = od.params(NX, NY, NZ, LX, LY, LZ, **workload_kwargs(od.OD_F32))
-    with od.Solver(p) as s:
-        s.set_state(u0)
-        dt = s.cfl_dt(CFL)
-        s.step(dt, args.warmup)
-        s.synchronize()
-        t0 = time.perf_counter()
-        ms = s.time_steps(dt, args.steps)
-        s.synchronize()
-        wall = time.perf_counter() - t0
-        kern = s.time_kernels(dt, 3)
-        info = s.info()
-        state_ok = bool(np.isfinite(s.get_state()).all())
+    # headline: fast_math = true, the reference's default (config.rs:89-91);
+    # the strict IEEE build (bit-identical to the oracle) is reported beside it
+    p = od.params(NX, NY, NZ, LX, LY, LZ, **workload_kwargs(od.OD_F32, fast_math=1))
+    p_strict = od.params(NX, NY, NZ, LX, LY, LZ, **workload_kwargs(od.OD_F32))
+    r = run_gpu(od, u0, p, args.steps, args.warmup)
+    rs = run_gpu(od, u0, p_strict, args.steps, args.warmup)
+    dt, ms, wall, kern, info, state_ok = r["dt"], r["ms"], r["wall"], r["kern"], r["info"], r["ok"]
     value = dof * args.steps / (ms / 1e3) / 1e9
     upd_ms = kern["od_update_kernel"]
     alg_bytes_launch = ALG_BYTES_PER_DOF_STEP / 3.0 * dof
@@ -144,6 +172,13 @@ def main():
         "kernel_ms": {k: round(v, 5) for k, v in kern.items()},
         "wall_s_timed": round(wall, 4),
         "state_finite": state_ok,
+        "fast_math": True,
+        "strict_ieee": {
+            "value": round(dof * args.steps / (rs["ms"] / 1e3) / 1e9, 4),
+            "ms_per_step": round(rs["ms"] / args.steps, 5),
+            "kernel_ms": {k: round(v, 5) for k, v in rs["kern"].items()},
+            "note": "fast_math = false: IEEE division/sqrt, no contraction; bitwise equal to the CPU oracle",
+        },
     }
     if args.cpu_steps > 0:
         from oracle import oracle as O
@@ -160,18 +195,9 @@ def main():
                       f"oracle, {threads} threads, {cpu_s:.1f} s",
         }
         out["speedup_vs_cpu"] = round(value / cpu_value, 1)
-        # per-variable error of the f32 GPU pipeline against the FP64 oracle
-        with od.Solver(p) as s:
-            s.set_state(u0)
-            s.step(dt, args.cpu_steps)
-            g = s.get_state().astype(np.float64).reshape(5, -1)
-        r = ref.reshape(5, -1)
-        names = ["rho", "rhou", "rhov", "rhow", "E"]
-        out["error_vs_fp64_oracle"] = {
-            "steps": args.cpu_steps,
-            "max_rel": {n: float(np.max(np.abs(g[v] - r[v])) / np.max(np.abs(r[v]))) for v, n in enumerate(names)},
-            "l2_rel": {n: float(np.linalg.norm(g[v] - r[v]) / np.linalg.norm(r[v])) for v, n in enumerate(names)},
-        }
+        # per-variable error of the f32 GPU pipelines against the FP64 oracle
+        out["error_vs_fp64_oracle"] = error_vs(ref, od, u0, p, dt, args.cpu_steps)
+        out["strict_ieee"]["error_vs_fp64_oracle"] = error_vs(ref, od, u0, p_strict, dt, args.cpu_steps)
     if args.copy_calibration:
         out["copy_calibration"] = hip_copy_calibration(1 << 30)
     print(json.dumps(out), flush=True)

@@ -63,7 +63,7 @@ typedef struct od_params {
     double mu_ref, t_ref, s_suth;
     double body_force_x;       /* [physics] body_force_x, reference value 0.001 */
     int32_t rk_accum;          /* [precision] rk_accum: 0 = compute type, 1 = state type */
-    int32_t fast_math;         /* [solver] fast_math (accepted; the IEEE path runs) */
+    int32_t fast_math;         /* [solver] fast_math: 1 = fast-math kernels for f32 compute */
 } od_params;
 
 typedef struct od_ctx od_ctx;

@@ -44,9 +44,14 @@ def _run(cmd):
 
 def build_lib():
     os.makedirs(OBJ, exist_ok=True)
-    srcs = ["od_kernels.hip", "od_api.hip"]
+    srcs = ["od_kernels.hip", "od_kernels_fast.hip", "od_api.hip"]
     objs = [os.path.join(OBJ, s.replace(".hip", ".o")) for s in srcs]
-    jobs = [[HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(srcs, objs)]
+    jobs = []
+    for s, o in zip(srcs, objs):
+        flags = list(HIP_FLAGS)
+        if s == "od_kernels_fast.hip":  # fast-math build: contraction allowed
+            flags = [f for f in flags if f != "-ffp-contract=off"] + ["-ffp-contract=fast"]
+        jobs.append([HIPCC, *flags, "-c", os.path.join(CSRC, s), "-o", o])
     with cf.ThreadPoolExecutor(len(jobs)) as ex:
         list(ex.map(_run, jobs))
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs])

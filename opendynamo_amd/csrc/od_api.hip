@@ -214,7 +214,14 @@ int launch_kind(od_ctx* c, int kind, int stage, double dt, int mode) {
     if (kind == 2) {
         a.uin = (const UT*)c->u[0];
         a.tr_out = (CT*)c->tr[0];
-        return od::launch_trace<UT, CT>(a, c->E, c->stream);
+        return od::launch_trace<UT, CT>(a, c->E, c->stream);  // pure data movement: one build serves both
+    }
+    if constexpr (sizeof(CT) == 4) {
+        if (c->p.fast_math) {
+            if (kind == 0) return od::fastmath::launch_visc<UT, CT>(a, c->E, c->stream);
+            if (c->p.viscous) return od::fastmath::launch_update<UT, CT, true>(a, c->E, c->stream);
+            return od::fastmath::launch_update<UT, CT, false>(a, c->E, c->stream);
+        }
     }
     if (kind == 0) return od::launch_visc<UT, CT>(a, c->E, c->stream);
     if (c->p.viscous) return od::launch_update<UT, CT, true>(a, c->E, c->stream);

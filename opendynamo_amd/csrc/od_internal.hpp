@@ -60,4 +60,15 @@ int launch_sums(const UT* u, double* partial, double* out, int64_t n_elem, const
 
 const char* kernel_name(int kind);
 
+// The same kernels built with fast math (f32 compute only): reciprocal
+// multiplies, hardware square root, contraction (od_kernels_fast.hip).
+namespace fastmath {
+template <class UT, class CT>
+int launch_trace(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+template <class UT, class CT>
+int launch_visc(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+template <class UT, class CT, bool VISC>
+int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+}  // namespace fastmath
+
 }  // namespace od

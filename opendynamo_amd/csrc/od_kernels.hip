@@ -25,8 +25,36 @@
 
 #include "od_internal.hpp"
 
+// OD_FAST = 1 builds the fast-math variant of the same kernels (selected by
+// the [solver] fast_math key for f32 compute): divisions become reciprocal
+// multiplies, square roots the hardware approximation, normal-vector zero
+// terms are dropped, and the translation unit is compiled with contraction.
+#ifndef OD_FAST
+#define OD_FAST 0
+#endif
+
 namespace od {
+#if OD_FAST
+namespace fastmath {
+#endif
 namespace {
+
+// a / b and sqrt(x): IEEE (correctly rounded) in the strict build; in the
+// fast build for f32, v_rcp_f32 / v_sqrt_f32 (1 ulp).
+template <class C>
+__device__ __forceinline__ C dv(C a, C b) {
+#if OD_FAST
+    if constexpr (sizeof(C) == 4) return a * __builtin_amdgcn_rcpf(b);
+#endif
+    return a / b;
+}
+template <class C>
+__device__ __forceinline__ C sq(C x) {
+#if OD_FAST
+    if constexpr (sizeof(C) == 4) return __builtin_amdgcn_sqrtf(x);
+#endif
+    return sqrt(x);
+}
 
 template <class T>
 __device__ __forceinline__ T sel4(T a0, T a1, T a2, T a3, int i) {
@@ -49,7 +77,7 @@ __device__ __forceinline__ C gamma_m1() { return ((C)1.4) - ((C)1.0); }
 
 template <class C>
 __device__ __forceinline__ C pressure(C rho, C ru, C rv, C rw, C e) {  // :1-5
-    C ke = ((C)0.5) * (ru * ru + rv * rv + rw * rw) / rho;
+    C ke = dv(((C)0.5) * (ru * ru + rv * rv + rw * rw), rho);
     C p = gamma_m1<C>() * (e - ke);
     return fmax(((C)1e-7), p);
 }
@@ -59,7 +87,7 @@ __device__ __forceinline__ void euler_flux(int d, const C* s, C* f) {  // :7-21
     C rho = s[0], e = s[4];
     C p = pressure(rho, s[1], s[2], s[3], e);
     C md = sel3(s[1], s[2], s[3], d);
-    C ud = md / rho;
+    C ud = dv(md, rho);
     f[0] = md;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -73,12 +101,12 @@ __device__ __forceinline__ void euler_flux(int d, const C* s, C* f) {  // :7-21
 template <class C>
 __device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu, C* f) {
     C rho = s[0];
-    C u[3] = {s[1] / rho, s[2] / rho, s[3] / rho};
+    C u[3] = {dv(s[1], rho), dv(s[2], rho), dv(s[3], rho)};
     C G[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) G[a][b] = (g[3 * (a + 1) + b] - u[a] * g[b]) / rho;
+        for (int b = 0; b < 3; ++b) G[a][b] = dv(g[3 * (a + 1) + b] - u[a] * g[b], rho);
     C div = G[0][0] + G[1][1] + G[2][2];
     const C two = (C)2.0, tt = ((C)2.0) / ((C)3.0);
     C t00 = mu * (two * G[0][0] - tt * div);
@@ -88,17 +116,17 @@ __device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu
     C t02 = mu * (G[0][2] + G[2][0]);
     C t12 = mu * (G[1][2] + G[2][1]);
     C r0 = sel3(t00, t01, t02, d), r1 = sel3(t01, t11, t12, d), r2 = sel3(t02, t12, t22, d);
-    C ke = ((C)0.5) * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / rho;
+    C ke = dv(((C)0.5) * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]), rho);
     C p = gamma_m1<C>() * (s[4] - ke);
     C gd = sel3(g[0], g[1], g[2], d);
     C gu = sel3(g[3], g[4], g[5], d);
     C gv = sel3(g[6], g[7], g[8], d);
     C gw = sel3(g[9], g[10], g[11], d);
     C ge = sel3(g[12], g[13], g[14], d);
-    C dke = (s[1] * gu + s[2] * gv + s[3] * gw) / rho - ke * gd / rho;
+    C dke = dv(s[1] * gu + s[2] * gv + s[3] * gw, rho) - dv(ke * gd, rho);
     C dp = gamma_m1<C>() * (ge - dke);
-    C coeff = mu * ((C)1.4) / (((C)0.72) * gamma_m1<C>());
-    C q = -coeff * (dp / rho - p * gd / (rho * rho));
+    C coeff = dv(mu * ((C)1.4), ((C)0.72) * gamma_m1<C>());
+    C q = -coeff * (dv(dp, rho) - dv(p * gd, rho * rho));
     f[0] = (C)0.0;
     f[1] = r0;
     f[2] = r1;
@@ -108,46 +136,67 @@ __device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu
 
 template <class C>
 __device__ __forceinline__ C sound_speed(C p, C rho) {
-    return sqrt(fmax(((C)0.0), ((C)1.4) * p / fmax(((C)1e-7), rho)));
+    return sq(fmax(((C)0.0), dv(((C)1.4) * p, fmax(((C)1e-7), rho))));
 }
 
-template <class C>
-__device__ __forceinline__ void rusanov(const C* l, const C* r, const C* n, C* f) {  // :104-128
-    C ul = l[1] / l[0], vl = l[2] / l[0], wl = l[3] / l[0];
-    C ur = r[1] / r[0], vr = r[2] / r[0], wr = r[3] / r[0];
+// Normal velocity u . n and momentum flux component m v_n + p n_c for the
+// face normal n = e_D.  The strict build keeps the reference's arithmetic
+// with the zero components (u*1 + v*0 + w*0, p*0), which is what fixes the
+// sign of zero results; the fast build drops the zero terms.
+template <int D, class C>
+__device__ __forceinline__ C nvel(C u, C v, C w) {
+#if OD_FAST
+    return D == 0 ? u : (D == 1 ? v : w);
+#else
+    const C n0 = D == 0 ? (C)1.0 : (C)0.0, n1 = D == 1 ? (C)1.0 : (C)0.0, n2 = D == 2 ? (C)1.0 : (C)0.0;
+    return u * n0 + v * n1 + w * n2;
+#endif
+}
+template <int D, class C>
+__device__ __forceinline__ C mflux(int c, C a, C b, C p) {
+#if OD_FAST
+    return c == D ? a * b + p : a * b;
+#else
+    return a * b + p * (c == D ? (C)1.0 : (C)0.0);
+#endif
+}
+
+template <int D, class C>
+__device__ __forceinline__ void rusanov(const C* l, const C* r, C* f) {  // :104-128
+    C ul = dv(l[1], l[0]), vl = dv(l[2], l[0]), wl = dv(l[3], l[0]);
+    C ur = dv(r[1], r[0]), vr = dv(r[2], r[0]), wr = dv(r[3], r[0]);
     C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
     C pr = pressure(r[0], r[1], r[2], r[3], r[4]);
-    C vnl = ul * n[0] + vl * n[1] + wl * n[2];
-    C vnr = ur * n[0] + vr * n[1] + wr * n[2];
+    C vnl = nvel<D>(ul, vl, wl);
+    C vnr = nvel<D>(ur, vr, wr);
     C cl = sound_speed(pl, l[0]), cr = sound_speed(pr, r[0]);
     C lam = fmax(fabs(vnl) + cl, fabs(vnr) + cr);
     C fl[5], fr[5];
     fl[0] = l[0] * vnl;
-    fl[1] = l[1] * vnl + pl * n[0];
-    fl[2] = l[2] * vnl + pl * n[1];
-    fl[3] = l[3] * vnl + pl * n[2];
-    fl[4] = vnl * (l[4] + pl);
     fr[0] = r[0] * vnr;
-    fr[1] = r[1] * vnr + pr * n[0];
-    fr[2] = r[2] * vnr + pr * n[1];
-    fr[3] = r[3] * vnr + pr * n[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        fl[1 + c] = mflux<D>(c, l[1 + c], vnl, pl);
+        fr[1 + c] = mflux<D>(c, r[1 + c], vnr, pr);
+    }
+    fl[4] = vnl * (l[4] + pl);
     fr[4] = vnr * (r[4] + pr);
 #pragma unroll
     for (int v = 0; v < 5; ++v) f[v] = ((C)0.5) * (fl[v] + fr[v] - lam * (r[v] - l[v]));
 }
 
-template <class C>
-__device__ __forceinline__ void ausm_plus(const C* l, const C* r, const C* n, C* f) {  // :130-188
-    C ul = l[1] / l[0], vl = l[2] / l[0], wl = l[3] / l[0];
-    C ur = r[1] / r[0], vr = r[2] / r[0], wr = r[3] / r[0];
+template <int D, class C>
+__device__ __forceinline__ void ausm_plus(const C* l, const C* r, C* f) {  // :130-188
+    C ul = dv(l[1], l[0]), vl = dv(l[2], l[0]), wl = dv(l[3], l[0]);
+    C ur = dv(r[1], r[0]), vr = dv(r[2], r[0]), wr = dv(r[3], r[0]);
     C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
     C pr = pressure(r[0], r[1], r[2], r[3], r[4]);
-    C vnl = ul * n[0] + vl * n[1] + wl * n[2];
-    C vnr = ur * n[0] + vr * n[1] + wr * n[2];
-    C hl = (l[4] + pl) / l[0], hr = (r[4] + pr) / r[0];
+    C vnl = nvel<D>(ul, vl, wl);
+    C vnr = nvel<D>(ur, vr, wr);
+    C hl = dv(l[4] + pl, l[0]), hr = dv(r[4] + pr, r[0]);
     C cl = sound_speed(pl, l[0]), cr = sound_speed(pr, r[0]);
     C cm = fmax(cl, cr);
-    C ml = vnl / cm, mr = vnr / cm;
+    C ml = dv(vnl, cm), mr = dv(vnr, cm);
     const C one = (C)1.0, q = (C)0.25, alpha = (C)0.1875, beta = (C)0.125, zero = (C)0.0;
     C mp, mm, pp, pm;
     if (fabs(ml) >= one) {
@@ -168,68 +217,68 @@ __device__ __forceinline__ void ausm_plus(const C* l, const C* r, const C* n, C*
     C p = pp * pl + pm * pr;
     bool pos = mach > zero;
     C mf = mach * cm * (pos ? l[0] : r[0]);
-    C us = pos ? ul : ur, vs = pos ? vl : vr, ws = pos ? wl : wr, hs = pos ? hl : hr;
+    C vs[3] = {pos ? ul : ur, pos ? vl : vr, pos ? wl : wr};
+    C hs = pos ? hl : hr;
     f[0] = mf;
-    f[1] = mf * us + p * n[0];
-    f[2] = mf * vs + p * n[1];
-    f[3] = mf * ws + p * n[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[1 + c] = mflux<D>(c, mf, vs[c], p);
     f[4] = mf * hs;
 }
 
-template <class C>
-__device__ __forceinline__ void slau2(const C* l, const C* r, const C* n, C* f) {  // :190-257
-    C ul = l[1] / l[0], vl = l[2] / l[0], wl = l[3] / l[0];
-    C ur = r[1] / r[0], vr = r[2] / r[0], wr = r[3] / r[0];
+template <int D, class C>
+__device__ __forceinline__ void slau2(const C* l, const C* r, C* f) {  // :190-257
+    C ul = dv(l[1], l[0]), vl = dv(l[2], l[0]), wl = dv(l[3], l[0]);
+    C ur = dv(r[1], r[0]), vr = dv(r[2], r[0]), wr = dv(r[3], r[0]);
     C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
     C pr = pressure(r[0], r[1], r[2], r[3], r[4]);
-    C vnl = ul * n[0] + vl * n[1] + wl * n[2];
-    C vnr = ur * n[0] + vr * n[1] + wr * n[2];
-    C hl = (l[4] + pl) / l[0], hr = (r[4] + pr) / r[0];
+    C vnl = nvel<D>(ul, vl, wl);
+    C vnr = nvel<D>(ur, vr, wr);
+    C hl = dv(l[4] + pl, l[0]), hr = dv(r[4] + pr, r[0]);
     C sql = ul * ul + vl * vl + wl * wl;
     C sqr = ur * ur + vr * vr + wr * wr;
     C cl = sound_speed(pl, l[0]), cr = sound_speed(pr, r[0]);
     const C half = (C)0.5, one = (C)1.0, zero = (C)0.0;
     C cf = half * (cl + cr);
-    C ml = vnl / cf, mr = vnr / cf;
-    C mt = fmin(one, (one / cf) * sqrt(half * (sql + sqr)));
+    C ml = dv(vnl, cf), mr = dv(vnr, cf);
+    C mt = fmin(one, dv(one, cf) * sq(half * (sql + sqr)));
     C chi = one - mt;
     chi = chi * chi;
     C g = -fmax(fmin(ml, zero), -one) * fmin(fmax(mr, zero), one);
-    C vavg = (l[0] * fabs(vnl) + r[0] * fabs(vnr)) / (l[0] + r[0]);
+    C vavg = dv(l[0] * fabs(vnl) + r[0] * fabs(vnr), l[0] + r[0]);
     C val = (one - g) * vavg + g * fabs(vnl);
     C var = (one - g) * vavg + g * fabs(vnr);
-    C mf = half * (l[0] * (vnl + val) + r[0] * (vnr - var) - (chi / cf) * (pr - pl));
+    C mf = half * (l[0] * (vnl + val) + r[0] * (vnr - var) - dv(chi, cf) * (pr - pl));
     C bl = (fabs(ml) < one) ? ((C)0.25) * (((C)2.0) - ml) * (ml + one) * (ml + one)
                             : ((ml >= zero) ? one : zero);
     C br = (fabs(mr) < one) ? ((C)0.25) * (((C)2.0) + mr) * (mr - one) * (mr - one)
                             : ((mr >= zero) ? zero : one);
     C pf = half * (pl + pr) + half * (bl - br) * (pl - pr);
-    pf += sqrt(half * (sql + sqr)) * (bl + br - one) * cf * half * (l[0] + r[0]);
+    pf += sq(half * (sql + sqr)) * (bl + br - one) * cf * half * (l[0] + r[0]);
     bool pos = mf > zero;
-    C us = pos ? ul : ur, vs = pos ? vl : vr, ws = pos ? wl : wr, hs = pos ? hl : hr;
+    C vs[3] = {pos ? ul : ur, pos ? vl : vr, pos ? wl : wr};
+    C hs = pos ? hl : hr;
     f[0] = mf;
-    f[1] = mf * us + pf * n[0];
-    f[2] = mf * vs + pf * n[1];
-    f[3] = mf * ws + pf * n[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[1 + c] = mflux<D>(c, mf, vs[c], pf);
     f[4] = mf * hs;
 }
 
-template <class C>
-__device__ __forceinline__ void interface_flux(int scheme, const C* l, const C* r, const C* n, C* f) {
+template <int D, class C>
+__device__ __forceinline__ void interface_flux(int scheme, const C* l, const C* r, C* f) {
     if (scheme == 0)
-        slau2(l, r, n, f);
+        slau2<D>(l, r, f);
     else if (scheme == 1)
-        ausm_plus(l, r, n, f);
+        ausm_plus<D>(l, r, f);
     else
-        rusanov(l, r, n, f);
+        rusanov<D>(l, r, f);
 }
 
 // Sutherland law, theta = gamma p / rho (no reference counterpart; the
 // reference parses the keys but uses a constant mu, constants.cuh:42).
 template <class C>
 __device__ __forceinline__ C sutherland(const C* s, C mu_ref, C t_ref, C s_suth) {
-    C theta = ((C)1.4) * pressure(s[0], s[1], s[2], s[3], s[4]) / s[0];
-    return ((mu_ref * (theta * sqrt(theta))) * (t_ref + s_suth)) / (t_ref * theta + s_suth);
+    C theta = dv(((C)1.4) * pressure(s[0], s[1], s[2], s[3], s[4]), s[0]);
+    return dv((mu_ref * (theta * sq(theta))) * (t_ref + s_suth), t_ref * theta + s_suth);
 }
 
 // ---------------------------------------------------------------------------
@@ -490,12 +539,11 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
         L[v] = right ? own[v] : oth[v];
         R[v] = right ? oth[v] : own[v];
     }
-    const CT nrm[3] = {D == 0 ? (CT)1.0 : (CT)0.0, D == 1 ? (CT)1.0 : (CT)0.0, D == 2 ? (CT)1.0 : (CT)0.0};
     CT fr[5], fe[5];
-    interface_flux(a.flux, L, R, nrm, fr);
+    interface_flux<D>(a.flux, L, R, fr);
     euler_flux(D, own, fe);
     CT wall_p = (CT)0;
-    if (D == 1) wall_p = fe[2] - own[2] * own[2] / own[0];  // rhs.cu:323-326
+    if (D == 1) wall_p = fe[2] - dv(own[2] * own[2], own[0]);  // rhs.cu:323-326
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const CT wf = (D == 1 && v == 2) ? wall_p : (CT)0.0;
@@ -791,6 +839,7 @@ int launch_sums(const UT* u, double* partial, double* out, int64_t n_elem, const
     return check_launch();
 }
 
+#if !OD_FAST
 const char* kernel_name(int kind) {
     switch (kind) {
         case 0: return "od_visc_kernel";
@@ -799,6 +848,7 @@ const char* kernel_name(int kind) {
         default: return "?";
     }
 }
+#endif
 
 #define OD_INST(UT, CT)                                                                  \
     template int launch_trace<UT, CT>(const StageArgs<UT, CT>&, int64_t, void*);         \
@@ -806,11 +856,18 @@ const char* kernel_name(int kind) {
     template int launch_update<UT, CT, true>(const StageArgs<UT, CT>&, int64_t, void*);  \
     template int launch_update<UT, CT, false>(const StageArgs<UT, CT>&, int64_t, void*);
 OD_INST(float, float)
-OD_INST(double, double)
 OD_INST(double, float)
+#if !OD_FAST
+OD_INST(double, double)
 OD_INST(float, double)
+#endif
 #undef OD_INST
+#if !OD_FAST
 template int launch_sums<float>(const float*, double*, double*, int64_t, const double*, double, void*);
 template int launch_sums<double>(const double*, double*, double*, int64_t, const double*, double, void*);
+#endif
 
+#if OD_FAST
+}  // namespace fastmath
+#endif
 }  // namespace od

@@ -181,3 +181,53 @@ def test_uniform_flow_free_stream():
     np.testing.assert_array_equal(r.ravel(), O.rhs_total(po, u))
     assert np.all(np.abs(r[1] - 0.001) < 1e-14)
     assert np.all(np.abs(r[[0, 2, 3, 4]]) < 1e-14)
+
+
+# ---------------------------------------------------------------------------
+# fast_math = true (the reference's default): f32 kernels with reciprocal
+# multiplies, hardware sqrt and contraction.  Not bitwise: checked against
+# the FP64 oracle with a stated tolerance, and for determinism / resume.
+# ---------------------------------------------------------------------------
+FAST_TOL = 2e-5  # max |gpu - oracle_f64| / max |oracle_f64| per variable
+
+
+@pytest.mark.parametrize("flux,wall,viscous,visc_model", CASES)
+def test_fast_math_against_fp64_oracle(flux, wall, viscous, visc_model):
+    nx, ny, nz = 3, 4, 2
+    lx, ly, lz = TWO_PI, 2.0, np.pi
+    announce(test="fast", flux=flux, wall=wall, viscous=viscous, visc_model=visc_model)
+    pg = od.params(nx, ny, nz, lx, ly, lz, u_real=4, flux=flux, y_wall=wall, viscous=viscous,
+                   viscosity_model=visc_model, mu_ref=2e-3, fast_math=1)
+    po = O.params(nx, ny, nz, lx, ly, lz, u_real=8, flux=flux, y_wall=wall, viscous=viscous,
+                  viscosity_model=visc_model, mu_ref=2e-3)
+    u = smooth_state(nx, ny, nz, lx, ly, lz, seed=5, amp=0.05, dtype=np.float32)
+    ref = O.step(po, u.astype(np.float64), 1e-3, 5)
+    with od.Solver(pg) as s:
+        s.set_state(u)
+        s.step(1e-3, 5)
+        got = s.get_state()
+    err = rel_err_per_var(got, ref)
+    print("fast-math rel err per var", err, flush=True)
+    assert np.all(err < FAST_TOL), err
+
+
+def test_fast_math_deterministic_and_resume():
+    nx, ny, nz = 4, 4, 4
+    lx, ly, lz = TWO_PI, 2.0, np.pi
+    announce(test="fast determinism")
+    pg = od.params(nx, ny, nz, lx, ly, lz, u_real=4, flux="slau2", y_wall=True, viscosity_model=1, mu_ref=1e-3,
+                   fast_math=1)
+    u = smooth_state(nx, ny, nz, lx, ly, lz, seed=1, amp=0.05, dtype=np.float32)
+    with od.Solver(pg) as s:
+        s.set_state(u)
+        s.step(1e-3, 20)
+        full = s.get_state()
+    with od.Solver(pg) as s:
+        s.set_state(u)
+        s.step(1e-3, 10)
+        mid = s.get_state()
+    with od.Solver(pg) as s:
+        s.set_state(mid)
+        s.step(1e-3, 10)
+        resumed = s.get_state()
+    assert resumed.tobytes() == full.tobytes()
