@@ -64,13 +64,17 @@ def build_host():
     if not srcs:
         return None
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HDF5_PREFIX, "include")]
-    link = ["-L" + os.path.join(HDF5_PREFIX, "lib"), "-Wl,-rpath," + os.path.join(HDF5_PREFIX, "lib"), "-lhdf5"]
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    # libhdf5 1.10 and libblosc (with lz4 / zstd) from the image's conda prefix
+    # the system libstdc++ must win over the older one in the conda prefix
+    link = ["-L" + lib, "-Wl,-rpath,/usr/lib/x86_64-linux-gnu:" + lib, "-lhdf5", "-lblosc"]
     _run(["g++", *CXX_FLAGS, "-shared", *inc, "-o", HOSTLIB + ".tmp", *srcs, *link])
     os.replace(HOSTLIB + ".tmp", HOSTLIB)
     main = os.path.join(HOST, "main.cpp")
     if os.path.exists(main):
+        rocm = "/opt/rocm/lib"
         _run(["g++", *CXX_FLAGS, *inc, "-o", CLI + ".tmp", main, "-L" + PKG, "-lodhost", "-lopendynamo_hip",
-              "-Wl,-rpath,$ORIGIN", *link])
+              "-Wl,-rpath,$ORIGIN", "-L" + rocm, "-Wl,-rpath," + rocm, "-Wl,--allow-shlib-undefined", *link])
         os.replace(CLI + ".tmp", CLI)
     return HOSTLIB
 

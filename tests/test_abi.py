@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(od_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(odh?_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -73,3 +73,12 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(od, "_lib", None)
     with pytest.raises(od.ODError):
         od.lib()
+
+
+def test_host_library_exports_every_declared_symbol():
+    from opendynamo_amd import host
+    names = [n for n in declared("odhost.h")]
+    lib = ctypes.CDLL(host.HOSTLIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(host.EXPORTS) == sorted(names)
