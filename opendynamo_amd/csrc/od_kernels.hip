@@ -23,6 +23,8 @@
 // kernel that computes it; comments cite the reference line.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "od_internal.hpp"
 
 // OD_FAST = 1 builds the fast-math variant of the same kernels (selected by
@@ -404,12 +406,81 @@ __device__ __forceinline__ int64_t tile_nb(const Tile& t, int le, int side) {
     return le ? n1 : n0;
 }
 
+// Neighbour across face f (0..5) of local element le, -1 = wall; selects on
+// named fields only.
+__device__ __forceinline__ int64_t elem_nb(const Elem& a, int f) {
+    return f == 0 ? a.xl : f == 1 ? a.xr : f == 2 ? a.yl : f == 3 ? a.yr : f == 4 ? a.zl : a.zr;
+}
+__device__ __forceinline__ int64_t tile_nb_rt(const Tile& t, int le, int f) {
+    const int64_t n0 = elem_nb(t.el0, f), n1 = elem_nb(t.el1, f);
+    return le ? n1 : n0;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-data prefetch (f32 compute).  At kernel start each work-group
+// copies everything it will read from its neighbours straight from global
+// memory into LDS with gfx950's global_load_lds_dwordx4 (no registers, one
+// exposed latency), as 16-byte chunks:
+//   A [0, 240)    neighbour facing traces, BR1 order, faces 0..5: 2 x 6 x 80
+//   B [240, 320)  neighbour facing x traces, sequential order:   2 x 2 x 80
+//   C [320, 512)  own face viscous fluxes fv:                    2 x 384
+//   D [512, 704)  neighbour facing fv:                           2 x 6 x 64
+// The float offsets of the four groups in the buffer:
+constexpr int kPfA = 0, kPfB = 960, kPfC = 1280, kPfD = 2048, kPfFloats = 2816;
+constexpr int kPfChunksVisc = 240, kPfChunksEuler = 320, kPfChunksFull = 704;
+
+// Source of chunk `id`; false for chunks of a missing second element (odd
+// nx), which are not copied.
+template <class UT, class CT>
+__device__ __forceinline__ bool prefetch_chunk(const StageArgs<UT, CT>& a, const Tile& tl, int id, const CT*& src) {
+    if (id < 240) {  // A
+        const int le = id / 120, f = (id % 120) / 20, c = id % 20;
+        if (!tile_valid(tl, le)) return false;
+        const int64_t own = tile_eidx(tl, le), nb = tile_nb_rt(tl, le, f);
+        src = a.tr_in + (nb >= 0 ? nb : own) * kTraceStride + (f ^ 1) * 80 + c * 4;
+    } else if (id < 320) {  // B
+        const int r = id - 240, le = r / 40, f = (r % 40) / 20, c = r % 20;
+        if (!tile_valid(tl, le)) return false;
+        const int64_t nb = tile_nb_rt(tl, le, f);  // x is periodic: always a neighbour
+        src = a.tr_in + nb * kTraceStride + (f == 0 ? 7 : 6) * 80 + c * 4;
+    } else if (id < 512) {  // C
+        const int r = id - 320, le = r / 96, c = r % 96;
+        if (!tile_valid(tl, le)) return false;
+        src = a.fv + tile_eidx(tl, le) * kFvStride + c * 4;
+    } else {  // D
+        const int r = id - 512, le = r / 96, f = (r % 96) / 16, c = r % 16;
+        if (!tile_valid(tl, le)) return false;
+        const int64_t own = tile_eidx(tl, le), nb = tile_nb_rt(tl, le, f);
+        src = a.fv + (nb >= 0 ? nb : own) * kFvStride + (f ^ 1) * 64 + c * 4;
+    }
+    return true;
+}
+
+// Issue the first n_chunks chunks; waves split the 64-chunk instructions.
+// The caller waits (s_waitcnt 0, then a barrier for the other wave's copies)
+// before reading the buffer.
+template <class UT, class CT>
+__device__ __forceinline__ void prefetch_issue(const StageArgs<UT, CT>& a, const Tile& tl, float* pf, int n_chunks,
+                                               int wave, int lane) {
+    const int n_inst = (n_chunks + 63) / 64;
+    const int first = wave == 0 ? 0 : (n_inst + 1) / 2;
+    const int last = wave == 0 ? (n_inst + 1) / 2 : n_inst;
+    for (int q = first; q < last; ++q) {
+        const int id = q * 64 + lane;
+        const CT* src = nullptr;
+        if (id < n_chunks && prefetch_chunk(a, tl, id, src)) {
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)(pf + q * 256), 16, 0, 0);
+        }
+    }
+}
+
 // BR1 jump 1/2 (u_nb + u_own) - u_own at every face point of direction D
 // (accumulate_jump, rhs.cu:105-108; wall ghost rhs.cu:111-114), once per face
 // point instead of once per lane of the normal line.
-template <int D, class UT, class CT>
+template <int D, bool PF, class UT, class CT>
 __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (*sU)[5][64], CT (*sJmp)[6][5][16],
-                                          const Tile& tl, int lane) {
+                                          const Tile& tl, const CT* pf, int lane) {
     const FaceTask t = face_task<D>(lane);
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
@@ -420,7 +491,12 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
     for (int v = 0; v < 5; ++v) {
         CT own = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, br1_order(D));
         const CT sign = (v > 0 && v < 4) ? ((CT)-1.0) : ((CT)1.0);
-        CT nbv = nb >= 0 ? a.tr_in[nb * kTraceStride + (t.f ^ 1) * 80 + v * 16 + t.fp] : own * sign;
+        CT far;
+        if constexpr (PF)
+            far = pf[kPfA + (t.le * 6 + t.f) * 80 + v * 16 + t.fp];
+        else
+            far = nb >= 0 ? a.tr_in[nb * kTraceStride + (t.f ^ 1) * 80 + v * 16 + t.fp] : (CT)0;
+        CT nbv = nb >= 0 ? far : own * sign;
         sJmp[t.le][t.f][v][t.fp] = ((CT)0.5) * (nbv + own) - own;
     }
 }
@@ -470,14 +546,15 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
     }
 }
 
-template <class UT, class CT>
+template <bool PF, class UT, class CT>
 __device__ __forceinline__ void run_br1_jumps(const StageArgs<UT, CT>& a, const CT (*sU)[5][64],
-                                              CT (*sJmp)[6][5][16], const Tile& tl, int wave, int lane) {
+                                              CT (*sJmp)[6][5][16], const Tile& tl, const CT* pf, int wave,
+                                              int lane) {
     if (wave == 0) {
-        br1_jumps<0>(a, sU, sJmp, tl, lane);
-        br1_jumps<2>(a, sU, sJmp, tl, lane);
+        br1_jumps<0, PF>(a, sU, sJmp, tl, pf, lane);
+        br1_jumps<2, PF>(a, sU, sJmp, tl, pf, lane);
     } else {
-        br1_jumps<1>(a, sU, sJmp, tl, lane);
+        br1_jumps<1, PF>(a, sU, sJmp, tl, pf, lane);
     }
 }
 
@@ -507,9 +584,9 @@ __device__ __forceinline__ void fv_pass(const StageArgs<UT, CT>& a, const CT (*s
 
 // Interface flux at the element faces of direction D -> own-side correction
 // j into LDS (compute_face_j / compute_face_j_wall, rhs.cu:273-339).
-template <int D, bool VISC, class UT, class CT>
+template <int D, bool VISC, bool PF, class UT, class CT>
 __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (*sU)[5][64], CT (*sJ)[6][5][16],
-                                          const Tile& tl, int lane) {
+                                          const Tile& tl, const CT* pf, int lane) {
     const FaceTask t = face_task<D>(lane);
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
@@ -524,14 +601,28 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
     const int rf = D == 0 ? (right ? 6 : 7) : (D == 1 ? (right ? 2 : 3) : (right ? 4 : 5));
 #pragma unroll
-    for (int v = 0; v < 5; ++v) oth[v] = wall ? own[v] : a.tr_in[nb * kTraceStride + rf * 80 + v * 16 + t.fp];
+    for (int v = 0; v < 5; ++v) {
+        CT far;
+        if constexpr (PF)
+            far = D == 0 ? pf[kPfB + (t.le * 2 + t.side) * 80 + v * 16 + t.fp]
+                         : pf[kPfA + (t.le * 6 + t.f) * 80 + v * 16 + t.fp];
+        else
+            far = wall ? own[v] : a.tr_in[nb * kTraceStride + rf * 80 + v * 16 + t.fp];
+        oth[v] = wall ? own[v] : far;
+    }
     CT fvo[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0}, fvn[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
     if (VISC) {
-        const CT* src_o = a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) fvo[1 + q] = src_o[q * 16];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) fvn[1 + q] = wall ? fvo[1 + q] : a.fv[nb * kFvStride + (t.f ^ 1) * 64 + q * 16 + t.fp];
+        for (int q = 0; q < 4; ++q) {
+            if constexpr (PF) {
+                fvo[1 + q] = pf[kPfC + t.le * kFvStride + t.f * 64 + q * 16 + t.fp];
+                const CT far = pf[kPfD + (t.le * 6 + t.f) * 64 + q * 16 + t.fp];
+                fvn[1 + q] = wall ? fvo[1 + q] : far;
+            } else {
+                fvo[1 + q] = a.fv[tile_eidx(tl, t.le) * kFvStride + t.f * 64 + q * 16 + t.fp];
+                fvn[1 + q] = wall ? fvo[1 + q] : a.fv[nb * kFvStride + (t.f ^ 1) * 64 + q * 16 + t.fp];
+            }
+        }
     }
     CT L[5], R[5];
 #pragma unroll
@@ -614,16 +705,20 @@ __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
     __shared__ CT sU[kEpb][5][64];
     __shared__ CT sG[kEpb][15][64];
     __shared__ CT sJmp[kEpb][6][5][16];
+    constexpr bool PF = std::is_same<CT, float>::value;
+    __shared__ float pf[PF ? kPfB : 4];
     const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
     const Tile tl = make_tile(a);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    if constexpr (PF) prefetch_issue(a, tl, pf, kPfChunksVisc, wave, pt);
     if (tile_valid(tl, wave)) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
     }
+    if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
-    run_br1_jumps(a, sU, sJmp, tl, wave, pt);
+    run_br1_jumps<PF>(a, sU, sJmp, tl, reinterpret_cast<const CT*>(pf), wave, pt);
     __syncthreads();
     if (tile_valid(tl, wave)) {
         CT g[15];
@@ -646,6 +741,9 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
     __shared__ CT sF[kEpb][5][64];
     __shared__ CT sJ[kEpb][6][5][16];
     __shared__ CT sJmp[kEpb][6][5][16];
+    constexpr bool PF = std::is_same<CT, float>::value;
+    __shared__ float pf[PF ? (VISC ? kPfFloats : kPfC) : 4];
+    const CT* pfc = reinterpret_cast<const CT*>(pf);
     const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
     const Tile tl = make_tile(a);
@@ -653,6 +751,7 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t gbase = tile_eidx(tl, wave) * 64 + pt;
     const Ops<CT>& o = a.ops;
+    if constexpr (PF) prefetch_issue(a, tl, pf, VISC ? kPfChunksFull : kPfChunksEuler, wave, pt);
 
     UT ucur[5], uprv[5];
 #pragma unroll
@@ -661,11 +760,12 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
         uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
         sU[wave][v][pt] = (CT)ucur[v];
     }
+    if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
 
     CT g[15];
     if (VISC) {
-        run_br1_jumps(a, sU, sJmp, tl, wave, pt);
+        run_br1_jumps<PF>(a, sU, sJmp, tl, pfc, wave, pt);
         __syncthreads();
         lane_gradient(a, sU[wave], sJmp[wave], i, j, k, g);
     }
@@ -719,10 +819,10 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
 
     // Interface fluxes at all own faces, direction by direction.
     if (wave == 0) {
-        flux_pass<0, VISC>(a, sU, sJ, tl, pt);
-        flux_pass<2, VISC>(a, sU, sJ, tl, pt);
+        flux_pass<0, VISC, PF>(a, sU, sJ, tl, pfc, pt);
+        flux_pass<2, VISC, PF>(a, sU, sJ, tl, pfc, pt);
     } else {
-        flux_pass<1, VISC>(a, sU, sJ, tl, pt);
+        flux_pass<1, VISC, PF>(a, sU, sJ, tl, pfc, pt);
     }
     __syncthreads();
 
