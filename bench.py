@@ -11,6 +11,15 @@ amplitude 0.05, seed 1), dt from CFL 0.3.
 
 A step is one full SSP-RK3 step (3 stages, one hipGraph launch) over the
 resident state.  Prints one JSON line.
+
+    python bench.py --workload ensemble [--steps 500]
+
+measures the ensemble instead: 24 independent periodic cubes [0, 2pi)^3 with
+16..40 elements per side (seed 7), P=3 NS f32 fast math, SLAU2, fixed mu
+1.6e-4, no body force, Mach 0.1 mean flow plus a divergence-free Fourier
+field per box (12..32 modes, amplitude 0.02..0.1, own seed), per-box dt from
+CFL 0.3; all boxes advance with one launch per kernel and stage, schedule
+picked by the autotuner.  Also one JSON line.
 """
 import argparse
 import json
@@ -117,6 +126,57 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
+def run_ensemble(args):
+    """The ensemble workload (see the module docstring)."""
+    import opendynamo_amd as od
+    from opendynamo_amd import ic
+
+    boxes = ic.ensemble_boxes()
+    p = od.params(1, 1, 1, 1.0, 1.0, 1.0, u_real=od.OD_F32, flux="slau2", y_wall=False, viscous=True,
+                  viscosity_model=0, mu_ref=0.00016, body_force_x=0.0, fast_math=1)
+    dims = [(b["nx"], b["ny"], b["nz"], b["lx"], b["ly"], b["lz"]) for b in boxes]
+    dof = sum(64 * b["nx"] * b["ny"] * b["nz"] for b in boxes)
+    with od.Batch(p, dims) as B:
+        for q, b in enumerate(boxes):
+            B.set_state(q, ic.ensemble_state(b, np.float32))
+        dt = B.cfl_dt(CFL)
+        B.synchronize()
+        best, best_ms, table = B.autotune(dt, steps_per_trial=3)
+        default_ms = [t["ms_per_step"] for t in table
+                      if t["orient"] == 0 and t["pairs"] == 1 and t["order"] == 0 and t["groups_per_cu"] == 0][0]
+        B.step(dt, args.warmup)
+        B.synchronize()
+        t0 = time.perf_counter()
+        ms = B.time_steps(dt, args.steps)
+        wall = time.perf_counter() - t0
+        per_box_steps = max(1, min(20, args.steps))
+        ms_per_box = B.time_steps(dt, per_box_steps, per_box=True) / per_box_steps
+        info = B.info()
+        finite = all(bool(np.isfinite(B.get_state(q)).all()) for q in range(len(boxes)))
+        sums = B.sums()
+    value = dof * args.steps / (ms / 1e3) / 1e9
+    out = {
+        "metric": "GDOF-steps/s, ensemble of 24 periodic boxes 16..40^3, P=3 NS f32", "value": round(value, 4),
+        "unit": "GDOF·steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": "24 independent periodic cubes [0,2pi)^3, n ~ U{16..40} elements per side (seed 7), "
+                        "P=3 NS f32 fast math, SLAU2, mu 1.6e-4, Mach 0.1 + div-free Fourier (12-32 modes, "
+                        "amp 0.02-0.1, per-box seed), per-box dt from CFL 0.3",
+            "boxes": [b["nx"] for b in boxes], "dof": dof, "launches_per_step": info["launches_per_step"],
+            "device_bytes": info["device_bytes"],
+        },
+        "schedule": best, "autotune_trials": len(table), "autotune_best_ms": round(best_ms, 5),
+        "default_schedule_ms": round(default_ms, 5),
+        "per_box_launches_ms_per_step": round(ms_per_box, 5),
+        "wall_s_timed": round(wall, 4), "state_finite": finite,
+        "mass_per_box_min_max": [float(sums[:, 0].min()), float(sums[:, 0].max())],
+        "autotune_table": table,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,11 +186,15 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy of known size, the reference "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
+    ap.add_argument("--workload", choices=["channel48", "ensemble"], default="channel48")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not a build).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble":
+        run_ensemble(args)
+        return
 
     import opendynamo_amd as od
     from opendynamo_amd import ic

@@ -22,7 +22,8 @@
  *                        stage update consumes, exposed for parity checks)
  *   od_destroy           drop of the CudaSlice buffers / context
  * Entry points without a reference counterpart: od_conservation_sums,
- * od_cfl_dt, od_time_steps, od_time_kernels, od_info.
+ * od_cfl_dt, od_time_steps, od_time_kernels, od_info, and the od_batch_*
+ * ensemble API (the reference runs one box per process).
  */
 #ifndef OPENDYNAMO_H
 #define OPENDYNAMO_H
@@ -118,6 +119,135 @@ od_status od_time_kernels(od_ctx* ctx, double dt, uint32_t n_steps, double out_m
 od_status od_info(od_ctx* ctx, double out[4]);
 
 const char* od_last_error(void);
+
+/* ---- Ensembles --------------------------------------------------------
+ * No reference counterpart.  A batch holds independent periodic or
+ * wall-bounded boxes of different sizes sharing one od_params (physics,
+ * scheme, precision; its nx..lz are ignored).  Every RK3 stage advances all
+ * boxes with one launch of the gradient kernel and one of the update kernel
+ * (a persistent grid walking the tiles of every box), captured as one
+ * hipGraph per step.  Each box's result is byte-identical to the same box
+ * run alone through od_step with the same dt, for every schedule. */
+typedef struct od_batch od_batch;
+
+typedef struct od_box {
+    int32_t nx, ny, nz;        /* element counts */
+    double lx, ly, lz;         /* extents */
+} od_box;
+
+/* How the batch kernels map tiles (two neighbouring elements, 128 threads)
+ * to work-groups.  The autotuner searches these; results never depend on it. */
+typedef struct od_schedule {
+    int32_t orient;            /* tile = element pair along 0 x, 1 y, 2 z */
+    int32_t pairs;             /* tiles per work-group: 1 (128 threads) or 2 (256) */
+    int32_t order;             /* 0 round robin; 1 contiguous tile range per XCD */
+    int32_t groups_per_cu;     /* 0: one work-group per work unit; k > 0: a
+                                  persistent grid of k (at most the resident
+                                  maximum) work-groups per CU */
+} od_schedule;
+
+od_status od_batch_create(const od_params* common, const od_box* boxes, int32_t n_boxes, int device,
+                          od_batch** out);
+void od_batch_destroy(od_batch* b);
+
+/* State of one box, layout as od_set_state with that box's dimensions. */
+od_status od_batch_set_state(od_batch* b, int32_t box, const void* host, size_t n_values, od_real dtype);
+od_status od_batch_get_state(od_batch* b, int32_t box, void* host, size_t n_values, od_real dtype);
+
+/* n_steps RK3 steps of every box; dt[box] is that box's step. Asynchronous. */
+od_status od_batch_step(od_batch* b, const double* dt, uint32_t n_steps);
+od_status od_batch_synchronize(od_batch* b);
+
+/* Per-box CFL step (as od_cfl_dt) -> dt_out[n_boxes];
+ * per-box GL integrals -> out[n_boxes][5]. */
+od_status od_batch_cfl_dt(od_batch* b, double cfl, double* dt_out);
+od_status od_batch_conservation_sums(od_batch* b, double* out);
+
+od_status od_batch_set_schedule(od_batch* b, const od_schedule* s);
+od_status od_batch_get_schedule(od_batch* b, od_schedule* s);
+
+/* Time every candidate schedule over steps_per_trial steps (after one
+ * untimed step) from the current state, restore the state after each trial,
+ * and keep the fastest.  Up to `cap` trials are reported in trials[] /
+ * trial_ms[] (ms per step); *n_trials = number of candidates timed. */
+od_status od_batch_autotune(od_batch* b, const double* dt, uint32_t steps_per_trial, od_schedule* best,
+                            double* best_ms, od_schedule* trials, double* trial_ms, int32_t cap,
+                            int32_t* n_trials);
+
+/* Time n_steps graph launches (ms total); per_box != 0 times the same steps
+ * launched box by box with the single-box kernels instead (6 launches per
+ * box per step, no graph), for comparison. */
+od_status od_batch_time_steps(od_batch* b, const double* dt, uint32_t n_steps, int32_t per_box, double* ms_total);
+
+/* out[0] = total DOF, out[1] = device bytes held, out[2] = kernel launches
+ * per step, out[3] = number of boxes. */
+od_status od_batch_info(od_batch* b, double out[4]);
+
+/* ---- Learned closure ---------------------------------------------------
+ * No reference counterpart.  An element-local MLP maps 12 invariants of the
+ * element's resolved field (features below) through 3 hidden layers of 64
+ * GELU units to a coefficient C = softplus(.) >= 0; the element's eddy
+ * viscosity mu_t = rho_e * Delta^2 * |S|_e * C is added to the molecular
+ * viscosity of every viscous flux of the element (mu = mu_mol + mu_t[e]).
+ * Weights and activations are BF16 on MFMA with FP32 accumulation.
+ * Features (element means with GL weights, Delta = element size / (P+1),
+ * |S| = sqrt(2 S^d:S^d)):
+ *   0 log(rho Delta^2 |S| / mu_mol)   1 |Omega| / |S|     2 Q / |S|^2
+ *   3 R / |S|^3                        4 div u / |S|       5..7 fraction of the
+ *   velocity fluctuation energy in Legendre modes of highest degree 1, 2, 3
+ *   8 rms(rho) / rho   9 rms(p) / p   10 Delta |S| / c   11 rms(u) / (Delta |S|)
+ * The feature kernel also returns rho_e, Delta^2 |S|_e, the element's
+ * kinetic energy and its integral of 2 S^d:S^d (16 floats per element). */
+#define OD_MLP_FEATURES 12
+#define OD_MLP_HIDDEN 64
+#define OD_FEATURE_STRIDE 16
+/* Parameter vector (f32): W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4 */
+#define OD_MLP_PARAMS 9217
+
+typedef struct od_mlp od_mlp;
+
+typedef struct od_closure_opts {
+    int32_t update_interval;   /* steps between eddy-viscosity updates (>= 1) */
+    double budget_kappa;       /* kinetic-energy budget check: when the energy the model
+                                  dissipated over the last interval exceeds kappa times the
+                                  observed kinetic-energy decay, or the kinetic energy grew,
+                                  the next interval runs the plain FR step; <= 0 disables it */
+} od_closure_opts;
+
+od_status od_mlp_create(int device, od_mlp** out);
+void od_mlp_destroy(od_mlp* m);
+od_status od_mlp_set_params(od_mlp* m, const float* params, size_t n);
+od_status od_mlp_get_params(od_mlp* m, float* params, size_t n);
+/* Input standardisation x' = (x - shift) * scale and the training range
+ * [lo, hi] of each raw feature: an element with a feature outside it gets
+ * mu_t = 0 (the plain FR step). */
+od_status od_mlp_set_scaling(od_mlp* m, const float shift[12], const float scale[12], const float lo[12],
+                             const float hi[12]);
+/* Forward pass on raw features x[n][12] -> C[n] (no range check). */
+od_status od_mlp_forward(od_mlp* m, const float* x, size_t n, float* y);
+/* Mean squared error of C against t[n] and its gradient (OD_MLP_PARAMS). */
+od_status od_mlp_loss_grad(od_mlp* m, const float* x, const float* t, size_t n, double* loss, float* grad);
+/* Adam (beta 0.9 / 0.999) on the device over `epochs` passes of shuffled
+ * mini-batches of (x, t); loss_per_epoch[epochs] may be NULL.  Deterministic
+ * for a given seed. */
+od_status od_mlp_train(od_mlp* m, const float* x, const float* t, size_t n, uint32_t epochs, uint32_t batch,
+                       float lr, uint64_t seed, float* loss_per_epoch);
+
+/* Features of every element of the current state: out[E][16]. */
+od_status od_closure_features(od_ctx* ctx, float* out, size_t n);
+/* Attach (m != NULL) or detach the network: od_step then refreshes mu_t
+ * from the state every update_interval steps (features -> MLP -> range
+ * check -> budget check).  The network must outlive the attachment. */
+od_status od_closure_attach(od_ctx* ctx, od_mlp* m, const od_closure_opts* opts);
+/* Set a per-element eddy viscosity host[E] directly, or switch it off
+ * (host == NULL: the plain FR step). */
+od_status od_set_eddy_viscosity(od_ctx* ctx, const void* host, size_t n, od_real dtype);
+od_status od_get_eddy_viscosity(od_ctx* ctx, void* host, size_t n, od_real dtype);
+/* out[0] updates, out[1] updates that fell back on the budget check,
+ * out[2] element-updates outside the training range, out[3] kinetic energy
+ * at the last update, out[4] model dissipation rate at the last update,
+ * out[5] 1 if the model is active for the current interval. */
+od_status od_closure_stats(od_ctx* ctx, double out[6]);
 
 #ifdef __cplusplus
 }

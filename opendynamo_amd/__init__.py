@@ -22,7 +22,16 @@ EXPORTS = [
     "od_get_state_async", "od_wait", "od_step", "od_synchronize", "od_rhs",
     "od_conservation_sums", "od_cfl_dt", "od_time_steps", "od_time_kernels", "od_info",
     "od_last_error",
+    "od_batch_create", "od_batch_destroy", "od_batch_set_state", "od_batch_get_state",
+    "od_batch_step", "od_batch_synchronize", "od_batch_cfl_dt", "od_batch_conservation_sums",
+    "od_batch_set_schedule", "od_batch_get_schedule", "od_batch_autotune", "od_batch_time_steps",
+    "od_batch_info",
+    "od_mlp_create", "od_mlp_destroy", "od_mlp_set_params", "od_mlp_get_params", "od_mlp_set_scaling",
+    "od_mlp_forward", "od_mlp_loss_grad", "od_mlp_train", "od_closure_features", "od_closure_attach",
+    "od_set_eddy_viscosity", "od_get_eddy_viscosity", "od_closure_stats",
 ]
+
+MLP_FEATURES, MLP_HIDDEN, FEATURE_STRIDE, MLP_PARAMS = 12, 64, 16, 9217
 
 
 class Params(ctypes.Structure):
@@ -38,6 +47,30 @@ class Params(ctypes.Structure):
         ("body_force_x", ctypes.c_double),
         ("rk_accum", ctypes.c_int32), ("fast_math", ctypes.c_int32),
     ]
+
+
+class Box(ctypes.Structure):
+    """Mirror of od_box."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+    ]
+
+
+class Schedule(ctypes.Structure):
+    """Mirror of od_schedule."""
+    _fields_ = [
+        ("orient", ctypes.c_int32), ("pairs", ctypes.c_int32),
+        ("order", ctypes.c_int32), ("groups_per_cu", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ClosureOpts(ctypes.Structure):
+    """Mirror of od_closure_opts."""
+    _fields_ = [("update_interval", ctypes.c_int32), ("budget_kappa", ctypes.c_double)]
 
 
 class ODError(RuntimeError):
@@ -75,8 +108,42 @@ def lib():
         L.od_info.argtypes = [vp, dp]
         L.od_last_error.argtypes = []
         L.od_last_error.restype = ctypes.c_char_p
+        sp = ctypes.POINTER(Schedule)
+        L.od_batch_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Box), ctypes.c_int32, ctypes.c_int,
+                                      ctypes.POINTER(vp)]
+        L.od_batch_destroy.argtypes = [vp]
+        L.od_batch_destroy.restype = None
+        L.od_batch_set_state.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_size_t, ctypes.c_int]
+        L.od_batch_get_state.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_size_t, ctypes.c_int]
+        L.od_batch_step.argtypes = [vp, dp, ctypes.c_uint32]
+        L.od_batch_synchronize.argtypes = [vp]
+        L.od_batch_cfl_dt.argtypes = [vp, ctypes.c_double, dp]
+        L.od_batch_conservation_sums.argtypes = [vp, dp]
+        L.od_batch_set_schedule.argtypes = [vp, sp]
+        L.od_batch_get_schedule.argtypes = [vp, sp]
+        L.od_batch_autotune.argtypes = [vp, dp, ctypes.c_uint32, sp, dp, sp, dp, ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int32)]
+        L.od_batch_time_steps.argtypes = [vp, dp, ctypes.c_uint32, ctypes.c_int32, dp]
+        L.od_batch_info.argtypes = [vp, dp]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.od_mlp_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.od_mlp_destroy.argtypes = [vp]
+        L.od_mlp_destroy.restype = None
+        L.od_mlp_set_params.argtypes = [vp, fp, ctypes.c_size_t]
+        L.od_mlp_get_params.argtypes = [vp, fp, ctypes.c_size_t]
+        L.od_mlp_set_scaling.argtypes = [vp, fp, fp, fp, fp]
+        L.od_mlp_forward.argtypes = [vp, fp, ctypes.c_size_t, fp]
+        L.od_mlp_loss_grad.argtypes = [vp, fp, fp, ctypes.c_size_t, dp, fp]
+        L.od_mlp_train.argtypes = [vp, fp, fp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                   ctypes.c_uint64, fp]
+        L.od_closure_features.argtypes = [vp, fp, ctypes.c_size_t]
+        L.od_closure_attach.argtypes = [vp, vp, ctypes.POINTER(ClosureOpts)]
+        L.od_set_eddy_viscosity.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+        L.od_get_eddy_viscosity.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+        L.od_closure_stats.argtypes = [vp, dp]
         for name in EXPORTS:
-            if name not in ("od_default_params", "od_destroy", "od_last_error"):
+            if name not in ("od_default_params", "od_destroy", "od_last_error", "od_batch_destroy",
+                            "od_mlp_destroy"):
                 getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -206,3 +273,233 @@ class Solver:
         out = np.zeros(4)
         _check(lib().od_info(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "od_info")
         return {"dof": int(out[0]), "device_bytes": int(out[1]), "launches_per_step": int(out[2])}
+
+    # ---- learned closure ----
+    @property
+    def n_elements(self):
+        return self.p.nx * self.p.ny * self.p.nz
+
+    def features(self):
+        """[E][16]: 12 features, rho_e, Delta^2 |S|_e, kinetic energy, integral of 2 S^d:S^d."""
+        out = np.empty((self.n_elements, FEATURE_STRIDE), dtype=np.float32)
+        _check(lib().od_closure_features(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size),
+               "od_closure_features")
+        return out
+
+    def set_eddy_viscosity(self, mu_t):
+        """Per-element eddy viscosity [E] (None switches it off)."""
+        if mu_t is None:
+            _check(lib().od_set_eddy_viscosity(self._h, None, 0, OD_F64), "od_set_eddy_viscosity")
+            return
+        a = np.ascontiguousarray(mu_t, dtype=np.float64)
+        _check(lib().od_set_eddy_viscosity(self._h, a.ctypes.data, a.size, OD_F64), "od_set_eddy_viscosity")
+
+    def get_eddy_viscosity(self):
+        out = np.empty(self.n_elements)
+        _check(lib().od_get_eddy_viscosity(self._h, out.ctypes.data, out.size, OD_F64), "od_get_eddy_viscosity")
+        return out
+
+    def attach_closure(self, mlp, update_interval=1, budget_kappa=1.0):
+        """Attach an MLP (None detaches); od_step refreshes mu_t every update_interval steps."""
+        if mlp is None:
+            _check(lib().od_closure_attach(self._h, None, None), "od_closure_attach")
+            self._mlp = None
+            return
+        o = ClosureOpts(int(update_interval), float(budget_kappa))
+        _check(lib().od_closure_attach(self._h, mlp._h, ctypes.byref(o)), "od_closure_attach")
+        self._mlp = mlp  # keep the network alive while attached
+
+    def closure_stats(self):
+        out = np.zeros(6)
+        _check(lib().od_closure_stats(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+               "od_closure_stats")
+        return {"updates": int(out[0]), "budget_fallbacks": int(out[1]), "out_of_range": int(out[2]),
+                "kinetic_energy": out[3], "model_dissipation": out[4], "active": bool(out[5])}
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Batch:
+    """An ensemble of independent boxes advanced together (od_batch_*).
+
+    `boxes` is a list of (nx, ny, nz, lx, ly, lz); `p` (from params()) gives
+    the shared physics, scheme and precision; its mesh fields are ignored.
+    """
+
+    def __init__(self, p, boxes, device=0):
+        self.p = p
+        self.boxes = [tuple(b) for b in boxes]
+        arr = (Box * len(self.boxes))()
+        for q, (nx, ny, nz, lx, ly, lz) in enumerate(self.boxes):
+            arr[q] = Box(nx, ny, nz, *(float(np.float32(x)) for x in (lx, ly, lz)))
+        self._h = ctypes.c_void_p()
+        _check(lib().od_batch_create(ctypes.byref(p), arr, len(self.boxes), int(device), ctypes.byref(self._h)),
+               "od_batch_create")
+
+    def close(self):
+        if self._h:
+            lib().od_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def dtype(self):
+        return _NP[self.p.u_real]
+
+    def n_values(self, box):
+        nx, ny, nz = self.boxes[box][:3]
+        return 5 * 64 * nx * ny * nz
+
+    def _dt(self, dt):
+        a = np.array([float(np.float32(x)) for x in np.broadcast_to(dt, (len(self.boxes),))], dtype=np.float64)
+        return a
+
+    def set_state(self, box, state):
+        a = np.ascontiguousarray(state)
+        if a.dtype not in (np.float32, np.float64):
+            a = a.astype(self.dtype)
+        code = OD_F32 if a.dtype == np.float32 else OD_F64
+        _check(lib().od_batch_set_state(self._h, int(box), a.ctypes.data, a.size, code), "od_batch_set_state")
+
+    def get_state(self, box, dtype=None):
+        dt = self.dtype if dtype is None else np.dtype(dtype).type
+        out = np.empty(self.n_values(box), dtype=dt)
+        code = OD_F32 if dt == np.float32 else OD_F64
+        _check(lib().od_batch_get_state(self._h, int(box), out.ctypes.data, out.size, code), "od_batch_get_state")
+        return out
+
+    def step(self, dt, n_steps=1):
+        """n RK3 steps of every box; dt is a scalar or one value per box (rounded to f32)."""
+        d = self._dt(dt)
+        _check(lib().od_batch_step(self._h, _dp(d), int(n_steps)), "od_batch_step")
+
+    def synchronize(self):
+        _check(lib().od_batch_synchronize(self._h), "od_batch_synchronize")
+
+    def cfl_dt(self, cfl):
+        out = np.zeros(len(self.boxes))
+        _check(lib().od_batch_cfl_dt(self._h, float(cfl), _dp(out)), "od_batch_cfl_dt")
+        return out
+
+    def sums(self):
+        out = np.zeros((len(self.boxes), 5))
+        _check(lib().od_batch_conservation_sums(self._h, _dp(out)), "od_batch_conservation_sums")
+        return out
+
+    def set_schedule(self, orient=0, pairs=1, order=0, groups_per_cu=0):
+        s = Schedule(orient, pairs, order, groups_per_cu)
+        _check(lib().od_batch_set_schedule(self._h, ctypes.byref(s)), "od_batch_set_schedule")
+
+    def get_schedule(self):
+        s = Schedule()
+        _check(lib().od_batch_get_schedule(self._h, ctypes.byref(s)), "od_batch_get_schedule")
+        return s.as_dict()
+
+    def autotune(self, dt, steps_per_trial=3, cap=256):
+        """Time every candidate schedule, keep the fastest; the state is unchanged."""
+        d = self._dt(dt)
+        best = Schedule()
+        best_ms = ctypes.c_double()
+        trials = (Schedule * cap)()
+        ms = np.zeros(cap)
+        n = ctypes.c_int32()
+        _check(lib().od_batch_autotune(self._h, _dp(d), int(steps_per_trial), ctypes.byref(best), ctypes.byref(best_ms),
+                                       trials, _dp(ms), cap, ctypes.byref(n)), "od_batch_autotune")
+        table = [dict(trials[q].as_dict(), ms_per_step=float(ms[q])) for q in range(min(n.value, cap))]
+        return best.as_dict(), best_ms.value, table
+
+    def time_steps(self, dt, n_steps, per_box=False):
+        d = self._dt(dt)
+        v = ctypes.c_double()
+        _check(lib().od_batch_time_steps(self._h, _dp(d), int(n_steps), int(bool(per_box)), ctypes.byref(v)),
+               "od_batch_time_steps")
+        return v.value
+
+    def info(self):
+        out = np.zeros(4)
+        _check(lib().od_batch_info(self._h, _dp(out)), "od_batch_info")
+        return {"dof": int(out[0]), "device_bytes": int(out[1]), "launches_per_step": int(out[2]),
+                "boxes": int(out[3])}
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class MLP:
+    """The closure network on the GPU (od_mlp_*): 12 -> 64 -> 64 -> 64 -> 1,
+    GELU hidden layers, softplus output, BF16 weights on MFMA."""
+
+    def __init__(self, device=0, params=None):
+        self._h = ctypes.c_void_p()
+        _check(lib().od_mlp_create(int(device), ctypes.byref(self._h)), "od_mlp_create")
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if self._h:
+            lib().od_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().od_mlp_set_params(self._h, _fp(a), a.size), "od_mlp_set_params")
+
+    def get_params(self):
+        out = np.empty(MLP_PARAMS, dtype=np.float32)
+        _check(lib().od_mlp_get_params(self._h, _fp(out), out.size), "od_mlp_get_params")
+        return out
+
+    def set_scaling(self, shift, scale, lo=None, hi=None):
+        f = lambda v, d: np.ascontiguousarray(np.full(MLP_FEATURES, d) if v is None else v, dtype=np.float32)
+        sh, sc, l, h = f(shift, 0.0), f(scale, 1.0), f(lo, -np.inf), f(hi, np.inf)
+        self._scaling = (sh, sc, l, h)
+        _check(lib().od_mlp_set_scaling(self._h, _fp(sh), _fp(sc), _fp(l), _fp(h)), "od_mlp_set_scaling")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        _check(lib().od_mlp_forward(self._h, _fp(x), x.shape[0], _fp(y)), "od_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().od_mlp_loss_grad(self._h, _fp(x), _fp(t), x.shape[0], ctypes.byref(loss), _fp(g)),
+               "od_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, epochs, batch=65536, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        hist = np.zeros(epochs, dtype=np.float32)
+        _check(lib().od_mlp_train(self._h, _fp(x), _fp(t), x.shape[0], int(epochs), int(batch), float(lr),
+                                  int(seed), _fp(hist)), "od_mlp_train")
+        return hist

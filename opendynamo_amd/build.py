@@ -44,7 +44,7 @@ def _run(cmd):
 
 def build_lib():
     os.makedirs(OBJ, exist_ok=True)
-    srcs = ["od_kernels.hip", "od_kernels_fast.hip", "od_api.hip"]
+    srcs = ["od_kernels.hip", "od_kernels_fast.hip", "od_api.hip", "od_batch.hip", "od_closure.hip"]
     objs = [os.path.join(OBJ, s.replace(".hip", ".o")) for s in srcs]
     jobs = []
     for s, o in zip(srcs, objs):

@@ -23,9 +23,9 @@
 #include <vector>
 
 #include "../../include/opendynamo.h"
-#include "od_internal.hpp"
+#include "od_ctx.hpp"
 
-namespace {
+namespace od::host {
 
 thread_local std::string g_err;
 
@@ -34,12 +34,7 @@ od_status fail(od_status s, const std::string& msg) {
     return s;
 }
 
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t _e = (expr);                                                                \
-        if (_e != hipSuccess)                                                                  \
-            return fail(OD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));       \
-    } while (0)
+const char* last_error() { return g_err.c_str(); }
 
 // ---- FR operators for P = 3 (fr_math.rs:3-142), f64 host arithmetic ------
 double legendre(int p, double x) {
@@ -59,10 +54,6 @@ double legendre_d(int p, double x) {
     double pf = (double)p;
     return pf / (x * x - 1.0) * (x * legendre(p, x) - legendre(p - 1, x));
 }
-
-struct HostOps {
-    double x[4], D[16], eL[4], eR[4], gL[4], gR[4], w[4];
-};
 
 HostOps fr_operators() {
     HostOps o{};
@@ -114,95 +105,30 @@ HostOps fr_operators() {
     return o;
 }
 
-template <class C>
-od::Ops<C> cast_ops(const HostOps& h) {
-    od::Ops<C> o;
-    for (int q = 0; q < 16; ++q) o.D[q] = (C)h.D[q];
-    for (int q = 0; q < 4; ++q) {
-        o.eL[q] = (C)h.eL[q];
-        o.eR[q] = (C)h.eR[q];
-        o.gL[q] = (C)h.gL[q];
-        o.gR[q] = (C)h.gR[q];
+double cfl_dt_from_state(const double* h, int64_t NP, double dx, double dy, double dz, double cfl) {
+    double lam = 0.0;
+    for (int64_t q = 0; q < NP; ++q) {
+        double r = h[q], ru = h[NP + q], rv = h[2 * NP + q], rw = h[3 * NP + q], e = h[4 * NP + q];
+        double ke = 0.5 * (ru * ru + rv * rv + rw * rw) / r;
+        double pp = std::fmax(1e-7, 0.4 * (e - ke));
+        double cs = std::sqrt(1.4 * pp / r);
+        double l = (std::fabs(ru / r) + cs) / dx + (std::fabs(rv / r) + cs) / dy + (std::fabs(rw / r) + cs) / dz;
+        lam = std::fmax(lam, l);
     }
-    return o;
+    return (double)(float)(cfl / (7.0 * lam));
 }
 
-}  // namespace
+}  // namespace od::host
 
-struct od_ctx {
-    od_params p;
-    int device = 0;
-    hipStream_t stream = nullptr, io_stream = nullptr;
-    int64_t E = 0, NP = 0;
-    size_t ubytes = 0, cbytes = 0;
-    void* u[2] = {nullptr, nullptr};
-    void* tr[3] = {nullptr, nullptr, nullptr};
-    void* fv = nullptr;
-    void* dbg = nullptr;
-    double* sums_partial = nullptr;
-    double* sums_out = nullptr;
-    HostOps ops;
-    bool state_set = false;
-    hipGraphExec_t graph = nullptr;
-    double graph_dt = 0.0;
-    void* pinned[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t slot_done[3] = {nullptr, nullptr, nullptr};
-    uint64_t next_ticket = 1;
-    uint64_t slot_ticket[3] = {0, 0, 0};
-    size_t device_bytes = 0;
-};
+using namespace od::host;
 
 namespace {
 
-struct Coeffs {
-    double a, b, cm;
-    bool stage1;
-};
-const Coeffs kStage[3] = {{1.0, 0.0, 1.0, true}, {0.75, 0.25, 0.25, false}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, false}};
-
 template <class UT, class CT>
 od::StageArgs<UT, CT> stage_args(od_ctx* c, int stage, double dt) {
-    od::StageArgs<UT, CT> a{};
-    const od_params& p = c->p;
-    UT* u0 = (UT*)c->u[0];
-    UT* u1 = (UT*)c->u[1];
-    a.uin = stage == 0 ? u0 : u1;
-    a.uprev = u0;
-    a.uout = stage == 2 ? u0 : u1;
-    a.tr_in = (const CT*)c->tr[stage];
-    a.tr_out = (CT*)c->tr[(stage + 1) % 3];
-    a.fv = (CT*)c->fv;
-    a.dbg = (CT*)c->dbg;
-    a.nx = p.nx;
-    a.ny = p.ny;
-    a.nz = p.nz;
-    a.wall = p.y_wall ? 1 : 0;
-    a.flux = p.flux_scheme;
-    a.sutherland = p.viscosity_model == 1 ? 1 : 0;
-    a.round_q = (sizeof(CT) == 8 && p.q_real == OD_F32) ? 1 : 0;
-    a.round_rhs = (sizeof(CT) == 8 && p.rhs_real == OD_F32) ? 1 : 0;
-    a.accum_state = p.rk_accum ? 1 : 0;
-    a.stage1 = kStage[stage].stage1 ? 1 : 0;
-    a.mode = 0;
-    // jac = 2 / (L / n) in the compute type (rhs.cu:366-369; rk.cu:93-95)
-    const CT lx = (CT)p.lx, ly = (CT)p.ly, lz = (CT)p.lz;
-    a.jx = ((CT)2.0) / (lx / ((CT)p.nx));
-    a.jy = ((CT)2.0) / (ly / ((CT)p.ny));
-    a.jz = ((CT)2.0) / (lz / ((CT)p.nz));
-    a.mu = (CT)p.mu_ref;
-    a.mu_ref = (CT)p.mu_ref;
-    a.t_ref = (CT)p.t_ref;
-    a.s_suth = (CT)p.s_suth;
-    a.bf = (CT)p.body_force_x;
-    // RK coefficients formed in the compute type from the f32 dt (solver.rs:256, 275-316)
-    const Coeffs& k = kStage[stage];
-    a.ca = (CT)k.a;
-    a.cb = (CT)k.b;
-    a.cc = (CT)k.cm * (CT)dt;
-    a.ua = (UT)k.a;
-    a.ub = (UT)k.b;
-    a.uc = (UT)k.cm * (UT)dt;
-    a.ops = cast_ops<CT>(c->ops);
+    od::StageArgs<UT, CT> a = make_stage_args<UT, CT>(c->p, c->p.nx, c->p.ny, c->p.nz, c->p.lx, c->p.ly, c->p.lz,
+                                                      c->u, c->tr, c->fv, c->dbg, stage, dt);
+    a.mu_t = c->mu_t_on ? (const CT*)c->mu_t : nullptr;
     return a;
 }
 
@@ -245,7 +171,7 @@ od_status enqueue_step(od_ctx* c, double dt) {
 }
 
 od_status ensure_graph(od_ctx* c, double dt) {
-    if (c->graph && c->graph_dt == dt) return OD_OK;
+    if (c->graph && c->graph_dt == dt && c->graph_mu_t == c->mu_t_on) return OD_OK;
     if (c->graph) {
         HIP_TRY(hipGraphExecDestroy(c->graph));
         c->graph = nullptr;
@@ -263,21 +189,15 @@ od_status ensure_graph(od_ctx* c, double dt) {
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     c->graph_dt = dt;
+    c->graph_mu_t = c->mu_t_on;
     return OD_OK;
-}
-
-bool real_ok(int r) { return r == OD_F32 || r == OD_F64; }
-
-template <class Dst, class Src>
-void convert(Dst* d, const Src* s, size_t n) {
-    for (size_t q = 0; q < n; ++q) d[q] = (Dst)s[q];
 }
 
 }  // namespace
 
 extern "C" {
 
-const char* od_last_error(void) { return g_err.c_str(); }
+const char* od_last_error(void) { return last_error(); }
 
 void od_default_params(od_params* p) {
     if (!p) return;
@@ -324,6 +244,14 @@ od_status od_create(const od_params* p, int device, od_ctx** out) {
     c->ubytes = (size_t)p->u_real;
     c->cbytes = (size_t)p->compute_real;
     c->ops = fr_operators();
+    {
+        const od::Ops<float> of = cast_ops<float>(c->ops);
+        const od::Ops<double> od64 = cast_ops<double>(c->ops);
+        if (od::upload_ops(of, od64) || od::fastmath::upload_ops(of, od64)) {
+            delete c;
+            return fail(OD_ERR_HIP, "copying the FR operators to the device failed");
+        }
+    }
     auto cleanup = [&](od_status s) {
         od_destroy(c);
         return s;
@@ -367,6 +295,7 @@ void od_destroy(od_ctx* c) {
     if (c->dbg) (void)hipFree(c->dbg);
     if (c->sums_partial) (void)hipFree(c->sums_partial);
     if (c->sums_out) (void)hipFree(c->sums_out);
+    od_closure_release_internal(c);
     for (int q = 0; q < 3; ++q) {
         if (c->pinned[q]) (void)hipHostFree(c->pinned[q]);
         if (c->slot_done[q]) (void)hipEventDestroy(c->slot_done[q]);
@@ -456,7 +385,14 @@ od_status od_step(od_ctx* c, double dt, uint32_t n_steps) {
     HIP_TRY(hipSetDevice(c->device));
     od_status s = ensure_graph(c, dt);
     if (s != OD_OK) return s;
-    for (uint32_t q = 0; q < n_steps; ++q) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    for (uint32_t q = 0; q < n_steps; ++q) {
+        if (c->mlp && c->steps_done % (uint64_t)c->closure.update_interval == 0) {
+            s = od_closure_update_internal(c, dt * c->closure.update_interval);
+            if (s != OD_OK) return s;
+        }
+        HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+        c->steps_done += 1;
+    }
     return OD_OK;
 }
 
@@ -509,18 +445,7 @@ od_status od_cfl_dt(od_ctx* c, double cfl, double* dt_out) {
     std::vector<double> h(n);
     od_status s = od_get_state(c, h.data(), n, OD_F64);
     if (s != OD_OK) return s;
-    const int64_t NP = c->NP;
-    const double dx = c->p.lx / c->p.nx, dy = c->p.ly / c->p.ny, dz = c->p.lz / c->p.nz;
-    double lam = 0.0;
-    for (int64_t q = 0; q < NP; ++q) {
-        double r = h[q], ru = h[NP + q], rv = h[2 * NP + q], rw = h[3 * NP + q], e = h[4 * NP + q];
-        double ke = 0.5 * (ru * ru + rv * rv + rw * rw) / r;
-        double pp = std::fmax(1e-7, 0.4 * (e - ke));
-        double cs = std::sqrt(1.4 * pp / r);
-        double l = (std::fabs(ru / r) + cs) / dx + (std::fabs(rv / r) + cs) / dy + (std::fabs(rw / r) + cs) / dz;
-        lam = std::fmax(lam, l);
-    }
-    *dt_out = (double)(float)(cfl / (7.0 * lam));
+    *dt_out = cfl_dt_from_state(h.data(), c->NP, c->p.lx / c->p.nx, c->p.ly / c->p.ny, c->p.lz / c->p.nz, cfl);
     return OD_OK;
 }
 

@@ -1,0 +1,47 @@
+// The single-box context (include/opendynamo.h), shared by the front-end
+// translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "od_host.hpp"
+
+struct od_ctx {
+    od_params p;
+    int device = 0;
+    hipStream_t stream = nullptr, io_stream = nullptr;
+    int64_t E = 0, NP = 0;
+    size_t ubytes = 0, cbytes = 0;
+    void* u[2] = {nullptr, nullptr};
+    void* tr[3] = {nullptr, nullptr, nullptr};
+    void* fv = nullptr;
+    void* dbg = nullptr;
+    double* sums_partial = nullptr;
+    double* sums_out = nullptr;
+    od::host::HostOps ops;
+    bool state_set = false;
+    hipGraphExec_t graph = nullptr;
+    double graph_dt = 0.0;
+    void* pinned[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t slot_done[3] = {nullptr, nullptr, nullptr};
+    uint64_t next_ticket = 1;
+    uint64_t slot_ticket[3] = {0, 0, 0};
+    size_t device_bytes = 0;
+    // learned closure: per-element eddy viscosity [E] in the compute type
+    void* mu_t = nullptr;
+    bool mu_t_on = false;
+    bool graph_mu_t = false;      // whether the captured graph reads mu_t
+    od_mlp* mlp = nullptr;        // attached network (not owned)
+    od_closure_opts closure{};    // attach options
+    uint64_t steps_done = 0;      // steps since attach (update cadence)
+    double ke_prev = -1.0;        // kinetic energy at the previous update
+    double eps_prev = 0.0;        // model dissipation rate applied since then
+    bool closure_active = false;  // false: budget fallback, plain FR until the next update
+    double closure_stats[6] = {0, 0, 0, 0, 0, 0};
+    void* feat = nullptr;         // [E][16] features, then eps [E] f32, out-of-range [E] int
+    double* red = nullptr;        // budget-check reductions (device)
+    double* red_host = nullptr;   // pinned copy
+};
+
+// Closure refresh and release (od_closure.hip), used by od_step / od_destroy.
+od_status od_closure_update_internal(od_ctx* c, double dt_interval);
+void od_closure_release_internal(od_ctx* c);

@@ -38,13 +38,41 @@ struct StageArgs {
     CT* tr_out;        // traces of uout
     CT* fv;            // face viscous fluxes of uin
     CT* dbg;           // mode 1: corrected RHS output [5][N]
+    const CT* mu_t;    // per-element eddy viscosity added to mu (null: plain FR)
     int32_t nx, ny, nz;
     int32_t wall, flux, sutherland, round_q, round_rhs, accum_state, stage1, mode;
     CT jx, jy, jz;
     CT mu, mu_ref, t_ref, s_suth, bf;
     CT ca, cb, cc;     // blend coefficients in the compute type
     UT ua, ub, uc;     // blend coefficients in the state type
-    Ops<CT> ops;
+};
+
+// Copy the FR operators to the kernels' constant memory on the current
+// device (both builds hold their own copy).
+int upload_ops(const Ops<float>& f32, const Ops<double>& f64);
+
+// One box of an ensemble batch.  The batch arrays hold the boxes back to
+// back: box b's elements are e_off .. e_off + E_b - 1 of the trace and fv
+// arrays, and its state [5][E_b][64] starts at 320 * e_off of each state
+// buffer (the per-box file layout, unchanged).
+template <class UT, class CT>
+struct BoxDev {
+    int64_t e_off;     // first element of the box
+    int32_t tile_off;  // first tile of the box in the batch tile numbering
+    int32_t nx, ny, nz;
+    int32_t pad;
+    CT jx, jy, jz;     // 2 / (L / n) in the compute type
+    CT cc[3];          // per-stage c * dt of the box in the compute type
+    UT uc[3];          // and in the state type
+};
+
+template <class UT, class CT>
+struct BatchArgs {
+    const BoxDev<UT, CT>* boxes;  // device array, sorted by tile_off
+    int32_t n_boxes, n_tiles, n_units;
+    int32_t orient;               // element pair direction of a tile: 0 x, 1 y, 2 z
+    int32_t order;                // 0 round robin, 1 contiguous range per XCD
+    int32_t stage;                // 0..2
 };
 
 // Kernel launchers (explicitly instantiated for UT, CT in {float, double}).
@@ -54,6 +82,13 @@ template <class UT, class CT>
 int launch_visc(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
 template <class UT, class CT, bool VISC>
 int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+template <class UT, class CT>
+int launch_batch_visc(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream);
+template <class UT, class CT, bool VISC>
+int launch_batch_update(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream);
+// Resident work-groups per CU of a batch kernel (kind 0 visc, 1 update).
+template <class UT, class CT>
+int batch_occupancy(int kind, int pairs, bool visc, int* groups_per_cu);
 template <class UT>
 int launch_sums(const UT* u, double* partial, double* out, int64_t n_elem, const double w3[64],
                 double jac, void* stream);
@@ -69,6 +104,13 @@ template <class UT, class CT>
 int launch_visc(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
 template <class UT, class CT, bool VISC>
 int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
+template <class UT, class CT>
+int launch_batch_visc(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream);
+template <class UT, class CT, bool VISC>
+int launch_batch_update(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream);
+template <class UT, class CT>
+int batch_occupancy(int kind, int pairs, bool visc, int* groups_per_cu);
+int upload_ops(const Ops<float>& f32, const Ops<double>& f64);
 }  // namespace fastmath
 
 }  // namespace od

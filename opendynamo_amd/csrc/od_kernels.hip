@@ -39,7 +39,22 @@ namespace od {
 #if OD_FAST
 namespace fastmath {
 #endif
+
+// FR operators in constant memory, set once per device by upload_ops (the
+// same P = 3 values for every solver): kept out of the kernel arguments so
+// they do not occupy scalar registers for the whole kernel.
+__constant__ Ops<float> g_ops_f32;
+__constant__ Ops<double> g_ops_f64;
+
 namespace {
+
+template <class C>
+__device__ __forceinline__ const Ops<C>& dev_ops() {
+    if constexpr (sizeof(C) == 4)
+        return g_ops_f32;
+    else
+        return g_ops_f64;
+}
 
 // a / b and sqrt(x): IEEE (correctly rounded) in the strict build; in the
 // fast build for f32, v_rcp_f32 / v_sqrt_f32 (1 ulp).
@@ -295,23 +310,23 @@ __device__ __forceinline__ C sutherland(const C* s, C mu_ref, C t_ref, C s_suth)
 constexpr int kEpb = 2;
 
 struct Elem {
-    int64_t xl, xr, yl, yr, zl, zr;  // neighbours across each face; -1 = slip wall
+    int32_t xl, xr, yl, yr, zl, zr;  // neighbours across each face; -1 = slip wall (box-local, < 2^31)
 };
 
 // Element (ex, ey, ez) and its face neighbours; coordinates come from the
 // launch grid, so no integer division is needed.
 __device__ __forceinline__ Elem make_elem(int ex, int ey, int ez, int nx, int ny, int nz, int wall) {
     Elem r;
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    const int64_t e = ex + sy * ey + sz * ez;
+    const int sy = nx, sz = nx * ny;
+    const int e = ex + sy * ey + sz * ez;
     r.xl = e + (ex == 0 ? nx - 1 : -1);
     r.xr = e + (ex == nx - 1 ? 1 - nx : 1);
-    const int64_t yl = e + (ey == 0 ? (int64_t)(ny - 1) * sy : -sy);
-    const int64_t yr = e + (ey == ny - 1 ? -(int64_t)(ny - 1) * sy : sy);
+    const int yl = e + (ey == 0 ? (ny - 1) * sy : -sy);
+    const int yr = e + (ey == ny - 1 ? -(ny - 1) * sy : sy);
     r.yl = (wall && ey == 0) ? -1 : yl;
     r.yr = (wall && ey == ny - 1) ? -1 : yr;
-    r.zl = e + (ez == 0 ? (int64_t)(nz - 1) * sz : -sz);
-    r.zr = e + (ez == nz - 1 ? -(int64_t)(nz - 1) * sz : sz);
+    r.zl = e + (ez == 0 ? (nz - 1) * sz : -sz);
+    r.zr = e + (ez == nz - 1 ? -(nz - 1) * sz : sz);
     return r;
 }
 
@@ -373,47 +388,70 @@ __device__ __forceinline__ FaceTask face_task(int lane) {
 
 // The work-group's two elements, held in named scalars (no private arrays).
 struct Tile {
-    int64_t e0, e1;
+    int32_t e0, e1;
     bool v0, v1;
     Elem el0, el1;
 };
 
-// Launch grid: blockIdx = (pair of x elements, ey, ez).
+// Tile (t0, t1, t2) of a box: two elements adjacent along `orient`
+// (0 x, 1 y, 2 z); the tile coordinate along `orient` counts pairs.
 template <class UT, class CT>
-__device__ __forceinline__ Tile make_tile(const StageArgs<UT, CT>& a) {
+__device__ __forceinline__ Tile make_tile_at(const StageArgs<UT, CT>& a, int orient, int t0, int t1, int t2) {
     Tile t;
-    const int ex0 = blockIdx.x * kEpb, ey = blockIdx.y, ez = blockIdx.z;
-    const int64_t row = (int64_t)a.nx * (ey + (int64_t)a.ny * ez);
-    t.e0 = row + ex0;
-    t.e1 = t.e0 + 1;
+    int ex = t0, ey = t1, ez = t2;
+    int step;
+    bool v1;
+    if (orient == 0) {
+        ex = kEpb * t0;
+        v1 = ex + 1 < a.nx;
+        step = 1;
+    } else if (orient == 1) {
+        ey = kEpb * t1;
+        v1 = ey + 1 < a.ny;
+        step = a.nx;
+    } else {
+        ez = kEpb * t2;
+        v1 = ez + 1 < a.nz;
+        step = a.nx * a.ny;
+    }
+    t.e0 = ex + a.nx * (ey + a.ny * ez);
+    t.e1 = t.e0 + step;
     t.v0 = true;
-    t.v1 = ex0 + 1 < a.nx;
-    t.el0 = make_elem(ex0, ey, ez, a.nx, a.ny, a.nz, a.wall);
-    t.el1 = make_elem(t.v1 ? ex0 + 1 : ex0, ey, ez, a.nx, a.ny, a.nz, a.wall);
+    t.v1 = v1;
+    t.el0 = make_elem(ex, ey, ez, a.nx, a.ny, a.nz, a.wall);
+    const int d = v1 ? 1 : 0;
+    t.el1 = make_elem(ex + (orient == 0 ? d : 0), ey + (orient == 1 ? d : 0), ez + (orient == 2 ? d : 0), a.nx, a.ny,
+                      a.nz, a.wall);
     return t;
 }
 
+// Single-box launch grid: blockIdx = (pair of x elements, ey, ez).
+template <class UT, class CT>
+__device__ __forceinline__ Tile make_tile(const StageArgs<UT, CT>& a) {
+    return make_tile_at(a, 0, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+
 __device__ __forceinline__ bool tile_valid(const Tile& t, int le) { return le ? t.v1 : t.v0; }
-__device__ __forceinline__ int64_t tile_eidx(const Tile& t, int le) { return le ? t.e1 : t.e0; }
+__device__ __forceinline__ int64_t tile_eidx(const Tile& t, int le) { return (int64_t)(le ? t.e1 : t.e0); }
 // Neighbour across the side `side` face of direction D of local element le.
 template <int D>
 __device__ __forceinline__ int64_t tile_nb(const Tile& t, int le, int side) {
     const Elem& a = t.el0;
     const Elem& b = t.el1;
-    int64_t l0 = D == 0 ? a.xl : (D == 1 ? a.yl : a.zl), r0 = D == 0 ? a.xr : (D == 1 ? a.yr : a.zr);
-    int64_t l1 = D == 0 ? b.xl : (D == 1 ? b.yl : b.zl), r1 = D == 0 ? b.xr : (D == 1 ? b.yr : b.zr);
-    int64_t n0 = side ? r0 : l0, n1 = side ? r1 : l1;
-    return le ? n1 : n0;
+    int l0 = D == 0 ? a.xl : (D == 1 ? a.yl : a.zl), r0 = D == 0 ? a.xr : (D == 1 ? a.yr : a.zr);
+    int l1 = D == 0 ? b.xl : (D == 1 ? b.yl : b.zl), r1 = D == 0 ? b.xr : (D == 1 ? b.yr : b.zr);
+    int n0 = side ? r0 : l0, n1 = side ? r1 : l1;
+    return (int64_t)(le ? n1 : n0);
 }
 
 // Neighbour across face f (0..5) of local element le, -1 = wall; selects on
 // named fields only.
-__device__ __forceinline__ int64_t elem_nb(const Elem& a, int f) {
+__device__ __forceinline__ int elem_nb(const Elem& a, int f) {
     return f == 0 ? a.xl : f == 1 ? a.xr : f == 2 ? a.yl : f == 3 ? a.yr : f == 4 ? a.zl : a.zr;
 }
 __device__ __forceinline__ int64_t tile_nb_rt(const Tile& t, int le, int f) {
-    const int64_t n0 = elem_nb(t.el0, f), n1 = elem_nb(t.el1, f);
-    return le ? n1 : n0;
+    const int n0 = elem_nb(t.el0, f), n1 = elem_nb(t.el1, f);
+    return (int64_t)(le ? n1 : n0);
 }
 
 // ---------------------------------------------------------------------------
@@ -485,7 +523,7 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
     face_line(D, t.fp, base, stride);
-    const Coef4<CT> c = side_coef(a.ops, t.side);
+    const Coef4<CT> c = side_coef(dev_ops<CT>(), t.side);
     const int64_t nb = tile_nb<D>(tl, t.le, t.side);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -506,7 +544,7 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
 template <class UT, class CT>
 __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const CT (*sU)[64], const CT (*sJmp)[5][16],
                                               int i, int j, int k, CT* g) {
-    const Ops<CT>& o = a.ops;
+    const Ops<CT>& o = dev_ops<CT>();
     CT Di[4], Dj[4], Dk[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -569,13 +607,14 @@ __device__ __forceinline__ void fv_pass(const StageArgs<UT, CT>& a, const CT (*s
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
     face_line(D, t.fp, base, stride);
-    const Coef4<CT> c = side_coef(a.ops, t.side);
+    const Coef4<CT> c = side_coef(dev_ops<CT>(), t.side);
     CT st[5], gt[15], fvv[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) st[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
 #pragma unroll
     for (int q = 0; q < 15; ++q) gt[q] = trace4(sG[t.le][q], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
     CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+    if (a.mu_t) mu = mu + a.mu_t[tile_eidx(tl, t.le)];
     viscous_flux(D, st, gt, mu, fvv);
     CT* dst = a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp;
 #pragma unroll
@@ -591,7 +630,7 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
     face_line(D, t.fp, base, stride);
-    const Coef4<CT> c = side_coef(a.ops, t.side);
+    const Coef4<CT> c = side_coef(dev_ops<CT>(), t.side);
     const bool right = t.side;
     CT own[5], oth[5];
 #pragma unroll
@@ -686,94 +725,106 @@ __device__ __forceinline__ void write_traces2(const Ops<CT>& o, const CT (*sU)[5
 // ---------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------
+// LDS of one tile (two elements).  The prefetch buffer comes first so that
+// its 16-byte LDS-DMA rows start on a 16-byte boundary.
+template <class CT>
+struct ViscLds {
+    float pf[std::is_same<CT, float>::value ? kPfB : 4];
+    CT sU[kEpb][5][64];
+    CT sG[kEpb][15][64];
+    CT sJmp[kEpb][6][5][16];
+};
+
+template <class CT, bool VISC>
+struct UpdateLds {
+    float pf[std::is_same<CT, float>::value ? (VISC ? kPfFloats : kPfC) : 4];
+    CT sU[kEpb][5][64];
+    CT sF[kEpb][5][64];
+    CT sJ[kEpb][6][5][16];
+    CT sJmp[kEpb][6][5][16];
+};
+
+// Face traces of the state -> tr_out (wave w loads element w of the tile).
 template <class UT, class CT>
-__global__ __launch_bounds__(128) void od_trace_kernel(StageArgs<UT, CT> a) {
-    __shared__ CT sU[kEpb][5][64];
-    const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
-    const Tile tl = make_tile(a);
+__device__ __forceinline__ void trace_body(const StageArgs<UT, CT>& a, const Tile& tl, CT (*sU)[5][64], int wave,
+                                           int pt) {
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     if (tile_valid(tl, wave)) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
     }
     __syncthreads();
-    write_traces2(a.ops, sU, a.tr_out, tl, wave, pt, true);
+    write_traces2(dev_ops<CT>(), sU, a.tr_out, tl, wave, pt, true);
 }
 
+// Corrected gradients of the tile's elements and the viscous fluxes at
+// their own faces -> fv.
 template <class UT, class CT>
-__global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
-    __shared__ CT sU[kEpb][5][64];
-    __shared__ CT sG[kEpb][15][64];
-    __shared__ CT sJmp[kEpb][6][5][16];
+__device__ __forceinline__ void visc_body(const StageArgs<UT, CT>& a, const Tile& tl, ViscLds<CT>& L, int wave,
+                                          int pt) {
     constexpr bool PF = std::is_same<CT, float>::value;
-    __shared__ float pf[PF ? kPfB : 4];
-    const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Tile tl = make_tile(a);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
-    if constexpr (PF) prefetch_issue(a, tl, pf, kPfChunksVisc, wave, pt);
+    if constexpr (PF) prefetch_issue(a, tl, L.pf, kPfChunksVisc, wave, pt);
     if (tile_valid(tl, wave)) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
+        for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
     }
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
-    run_br1_jumps<PF>(a, sU, sJmp, tl, reinterpret_cast<const CT*>(pf), wave, pt);
+    run_br1_jumps<PF>(a, L.sU, L.sJmp, tl, reinterpret_cast<const CT*>(L.pf), wave, pt);
     __syncthreads();
     if (tile_valid(tl, wave)) {
         CT g[15];
-        lane_gradient(a, sU[wave], sJmp[wave], i, j, k, g);
+        lane_gradient(a, L.sU[wave], L.sJmp[wave], i, j, k, g);
 #pragma unroll
-        for (int q = 0; q < 15; ++q) sG[wave][q][pt] = g[q];
+        for (int q = 0; q < 15; ++q) L.sG[wave][q][pt] = g[q];
     }
     __syncthreads();
     if (wave == 0) {
-        fv_pass<0>(a, sU, sG, tl, pt);
-        fv_pass<2>(a, sU, sG, tl, pt);
+        fv_pass<0>(a, L.sU, L.sG, tl, pt);
+        fv_pass<2>(a, L.sU, L.sG, tl, pt);
     } else {
-        fv_pass<1>(a, sU, sG, tl, pt);
+        fv_pass<1>(a, L.sU, L.sG, tl, pt);
     }
 }
 
+// Full stage for the tile: volume RHS, interface fluxes, FR correction, RK
+// blend, and the face traces of the new state.
 template <class UT, class CT, bool VISC>
-__global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
-    __shared__ CT sU[kEpb][5][64];
-    __shared__ CT sF[kEpb][5][64];
-    __shared__ CT sJ[kEpb][6][5][16];
-    __shared__ CT sJmp[kEpb][6][5][16];
+__device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Tile& tl, UpdateLds<CT, VISC>& L,
+                                            int wave, int pt) {
     constexpr bool PF = std::is_same<CT, float>::value;
-    __shared__ float pf[PF ? (VISC ? kPfFloats : kPfC) : 4];
-    const CT* pfc = reinterpret_cast<const CT*>(pf);
-    const int wave = threadIdx.x >> 6, pt = threadIdx.x & 63;
+    const CT* pfc = reinterpret_cast<const CT*>(L.pf);
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Tile tl = make_tile(a);
     const bool mine = tile_valid(tl, wave);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t gbase = tile_eidx(tl, wave) * 64 + pt;
-    const Ops<CT>& o = a.ops;
-    if constexpr (PF) prefetch_issue(a, tl, pf, VISC ? kPfChunksFull : kPfChunksEuler, wave, pt);
+    const Ops<CT>& o = dev_ops<CT>();
+    if constexpr (PF) prefetch_issue(a, tl, L.pf, VISC ? kPfChunksFull : kPfChunksEuler, wave, pt);
 
     UT ucur[5], uprv[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         ucur[v] = mine ? a.uin[v * NP + gbase] : (UT)1;
         uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
-        sU[wave][v][pt] = (CT)ucur[v];
+        L.sU[wave][v][pt] = (CT)ucur[v];
     }
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
 
     CT g[15];
     if (VISC) {
-        run_br1_jumps<PF>(a, sU, sJmp, tl, pfc, wave, pt);
+        run_br1_jumps<PF>(a, L.sU, L.sJmp, tl, pfc, wave, pt);
         __syncthreads();
-        lane_gradient(a, sU[wave], sJmp[wave], i, j, k, g);
+        lane_gradient(a, L.sU[wave], L.sJmp[wave], i, j, k, g);
     }
     CT st[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) st[v] = sU[wave][v][pt];
+    for (int v = 0; v < 5; ++v) st[v] = L.sU[wave][v][pt];
     CT mu = (CT)0;
     if (VISC) mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+    if (VISC && a.mu_t) mu = mu + (mine ? a.mu_t[tile_eidx(tl, wave)] : (CT)0);
 
     // Volume flux divergence, x then y then z (rhs.cu:229-266, 490-510).
     CT rs[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
@@ -792,7 +843,7 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
         }
         if (d > 0) __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 5; ++v) sF[wave][v][pt] = F[v];
+        for (int v = 0; v < 5; ++v) L.sF[wave][v][pt] = F[v];
         __syncthreads();
         const int idx = d == 0 ? i : (d == 1 ? j : k);
         const int base = d == 0 ? 4 * j + 16 * k : (d == 1 ? i + 16 * k : i + 4 * j);
@@ -805,7 +856,7 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
         for (int v = 0; v < 5; ++v) {
             CT dd = 0;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) dd += Dd[m] * sF[wave][v][base + m * stride];
+            for (int m = 0; m < 4; ++m) dd += Dd[m] * L.sF[wave][v][base + m * stride];
             rs[v] += dd * jac;
         }
     }
@@ -819,10 +870,10 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
 
     // Interface fluxes at all own faces, direction by direction.
     if (wave == 0) {
-        flux_pass<0, VISC, PF>(a, sU, sJ, tl, pfc, pt);
-        flux_pass<2, VISC, PF>(a, sU, sJ, tl, pfc, pt);
+        flux_pass<0, VISC, PF>(a, L.sU, L.sJ, tl, pfc, pt);
+        flux_pass<2, VISC, PF>(a, L.sU, L.sJ, tl, pfc, pt);
     } else {
-        flux_pass<1, VISC, PF>(a, sU, sJ, tl, pfc, pt);
+        flux_pass<1, VISC, PF>(a, L.sU, L.sJ, tl, pfc, pt);
     }
     __syncthreads();
 
@@ -834,7 +885,7 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
     UT unew[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        const CT(*J)[5][16] = sJ[wave];
+        const CT(*J)[5][16] = L.sJ[wave];
         CT cx = -(J[0][v][fpx] * gLi * a.jx) - (J[1][v][fpx] * gRi * a.jx);
         CT cy = -(J[2][v][fpy] * gLj * a.jy) - (J[3][v][fpy] * gRj * a.jy);
         CT cz = -(J[4][v][fpz] * gLk * a.jz) - (J[5][v][fpz] * gRk * a.jz);
@@ -859,9 +910,156 @@ __global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
     // Face traces of the new state for the next stage.
     __syncthreads();
 #pragma unroll
-    for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)unew[v];
+    for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)unew[v];
     __syncthreads();
-    write_traces2(o, sU, a.tr_out, tl, wave, pt, VISC);
+    write_traces2(o, L.sU, a.tr_out, tl, wave, pt, VISC);
+}
+
+template <class UT, class CT>
+__global__ __launch_bounds__(128) void od_trace_kernel(StageArgs<UT, CT> a) {
+    __shared__ CT sU[kEpb][5][64];
+    trace_body(a, make_tile(a), sU, threadIdx.x >> 6, threadIdx.x & 63);
+}
+
+template <class UT, class CT>
+__global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
+    __shared__ ViscLds<CT> L;
+    visc_body(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
+}
+
+template <class UT, class CT, bool VISC>
+__global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
+    __shared__ UpdateLds<CT, VISC> L;
+    update_body<UT, CT, VISC>(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: the tiles of every box of a batch in one launch per phase.  A
+// work unit is `PAIRS` consecutive tiles (one 128-thread tile each); work-
+// groups walk the units in a fixed order, so every unit is done exactly once
+// and the result of each element does not depend on the schedule.
+// ---------------------------------------------------------------------------
+
+// Unit handled by this work-group at iteration `it`, or -1 when it is done.
+// order 0: round robin over all units.  order 1: work-groups are dispatched
+// round robin over the 8 XCDs, so XCD x = blockIdx.x % 8 takes the x-th
+// contiguous eighth of the units (its neighbours' traces stay in its L2).
+__device__ __forceinline__ int sched_unit(int order, int n_units, int it) {
+    if (order == 0) {
+        const int u = (int)blockIdx.x + it * (int)gridDim.x;
+        return u < n_units ? u : -1;
+    }
+    const int m = (int)gridDim.x >> 3, x = (int)blockIdx.x & 7, s = (int)blockIdx.x >> 3;
+    const int chunk = (n_units + 7) >> 3;
+    const int r0 = x * chunk, r1 = min(n_units, r0 + chunk);
+    const int u = r0 + s + it * m;
+    return u < r1 ? u : -1;
+}
+
+// Wave-uniform copies (SGPRs) of values loaded from the box table: the
+// loads sit after stores of earlier units, so the compiler cannot use
+// scalar loads by itself.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+__device__ __forceinline__ int64_t uni(int64_t x) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double uni(double x) { return __longlong_as_double(uni((int64_t)__double_as_longlong(x))); }
+__device__ __forceinline__ Elem uni(const Elem& e) {
+    return Elem{uni(e.xl), uni(e.xr), uni(e.yl), uni(e.yr), uni(e.zl), uni(e.zr)};
+}
+
+// Stage arguments and tile of batch tile t (t >= n_tiles: an empty tile).
+template <class UT, class CT>
+__device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int t,
+                                           StageArgs<UT, CT>& ab, Tile& tl) {
+    const bool ok = t < b.n_tiles;
+    const int tt = ok ? t : 0;
+    int lo = 0, hi = b.n_boxes - 1;
+    while (lo < hi) {  // last box whose first tile is <= tt
+        const int mid = (lo + hi + 1) >> 1;
+        if (uni(b.boxes[mid].tile_off) <= tt)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BoxDev<UT, CT>& bx = b.boxes[lo];
+    ab = a;
+    ab.nx = uni(bx.nx);
+    ab.ny = uni(bx.ny);
+    ab.nz = uni(bx.nz);
+    ab.jx = uni(bx.jx);
+    ab.jy = uni(bx.jy);
+    ab.jz = uni(bx.jz);
+    ab.cc = uni(bx.cc[b.stage]);
+    ab.uc = uni(bx.uc[b.stage]);
+    const int64_t e_off = uni(bx.e_off);
+    const int tile_off = uni(bx.tile_off);
+    const int64_t so = e_off * 320;  // box state [5][E_box][64] starts at 5 * 64 * e_off
+    ab.uin = a.uin + so;
+    ab.uprev = a.uprev + so;
+    ab.uout = a.uout + so;
+    ab.dbg = a.dbg ? a.dbg + so : a.dbg;
+    ab.tr_in = a.tr_in + e_off * kTraceStride;
+    ab.tr_out = a.tr_out + e_off * kTraceStride;
+    ab.fv = a.fv ? a.fv + e_off * kFvStride : a.fv;
+    ab.mu_t = a.mu_t ? a.mu_t + e_off : a.mu_t;
+    const int lt = tt - tile_off;
+    const int d0 = b.orient == 0 ? (ab.nx + kEpb - 1) / kEpb : ab.nx;
+    const int d1 = b.orient == 1 ? (ab.ny + kEpb - 1) / kEpb : ab.ny;
+    const int t0 = uni(lt % d0), r = uni(lt / d0);
+    tl = make_tile_at(ab, b.orient, t0, uni(r % d1), uni(r / d1));
+    tl.e0 = uni(tl.e0);
+    tl.e1 = uni(tl.e1);
+    tl.el0 = uni(tl.el0);
+    tl.el1 = uni(tl.el1);
+    tl.v0 = ok;
+    tl.v1 = ok && tl.v1;
+}
+
+template <class UT, class CT, int PAIRS>
+__global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
+    __shared__ ViscLds<CT> L[PAIRS];
+    const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
+    const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
+    for (int it = 0;; ++it) {
+        const int unit = sched_unit(b.order, b.n_units, it);
+        if (unit < 0) break;
+        // A fresh copy of the lane index each unit keeps the compiler from
+        // hoisting lane-dependent values out of the loop (they would stay
+        // live in registers across the whole kernel).
+        int wv = wave, pv = pt;
+        asm volatile("" : "+v"(wv), "+v"(pv));
+        StageArgs<UT, CT> ab;
+        Tile tl;
+        batch_tile(a, b, unit * PAIRS + pair, ab, tl);
+        visc_body(ab, tl, L[pair], wv, pv);
+        __syncthreads();  // LDS is reused by the next unit
+    }
+}
+
+template <class UT, class CT, bool VISC, int PAIRS>
+__global__ __launch_bounds__(128 * PAIRS) void od_batch_update_kernel(StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
+    __shared__ UpdateLds<CT, VISC> L[PAIRS];
+    const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
+    const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
+    for (int it = 0;; ++it) {
+        const int unit = sched_unit(b.order, b.n_units, it);
+        if (unit < 0) break;
+        // A fresh copy of the lane index each unit keeps the compiler from
+        // hoisting lane-dependent values out of the loop (they would stay
+        // live in registers across the whole kernel).
+        int wv = wave, pv = pt;
+        asm volatile("" : "+v"(wv), "+v"(pv));
+        StageArgs<UT, CT> ab;
+        Tile tl;
+        batch_tile(a, b, unit * PAIRS + pair, ab, tl);
+        update_body<UT, CT, VISC>(ab, tl, L[pair], wv, pv);
+        __syncthreads();  // LDS is reused by the next unit
+    }
 }
 
 // Conservation sums: per-element GL-weighted sums (sequential over points),
@@ -926,6 +1124,45 @@ int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
     return check_launch();
 }
 
+template <class UT, class CT>
+int launch_batch_visc(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream) {
+    if (pairs == 2)
+        hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 2>), dim3(groups), dim3(256), 0, (hipStream_t)stream, a, b);
+    else
+        hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 1>), dim3(groups), dim3(128), 0, (hipStream_t)stream, a, b);
+    return check_launch();
+}
+
+template <class UT, class CT, bool VISC>
+int launch_batch_update(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream) {
+    if (pairs == 2)
+        hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 2>), dim3(groups), dim3(256), 0, (hipStream_t)stream,
+                           a, b);
+    else
+        hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 1>), dim3(groups), dim3(128), 0, (hipStream_t)stream,
+                           a, b);
+    return check_launch();
+}
+
+template <class UT, class CT>
+int batch_occupancy(int kind, int pairs, bool visc, int* groups_per_cu) {
+    int n = 0;
+    hipError_t e;
+    if (kind == 0)
+        e = pairs == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_visc_kernel<UT, CT, 2>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_visc_kernel<UT, CT, 1>, 128, 0);
+    else if (visc)
+        e = pairs == 2
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, true, 2>, 256, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, true, 1>, 128, 0);
+    else
+        e = pairs == 2
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, false, 2>, 256, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, false, 1>, 128, 0);
+    *groups_per_cu = n;
+    return e == hipSuccess ? 0 : 1;
+}
+
 template <class UT>
 int launch_sums(const UT* u, double* partial, double* out, int64_t n_elem, const double w3[64], double jac,
                 void* stream) {
@@ -950,11 +1187,17 @@ const char* kernel_name(int kind) {
 }
 #endif
 
-#define OD_INST(UT, CT)                                                                  \
-    template int launch_trace<UT, CT>(const StageArgs<UT, CT>&, int64_t, void*);         \
-    template int launch_visc<UT, CT>(const StageArgs<UT, CT>&, int64_t, void*);          \
-    template int launch_update<UT, CT, true>(const StageArgs<UT, CT>&, int64_t, void*);  \
-    template int launch_update<UT, CT, false>(const StageArgs<UT, CT>&, int64_t, void*);
+#define OD_INST(UT, CT)                                                                                      \
+    template int launch_trace<UT, CT>(const StageArgs<UT, CT>&, int64_t, void*);                             \
+    template int launch_visc<UT, CT>(const StageArgs<UT, CT>&, int64_t, void*);                              \
+    template int launch_update<UT, CT, true>(const StageArgs<UT, CT>&, int64_t, void*);                      \
+    template int launch_update<UT, CT, false>(const StageArgs<UT, CT>&, int64_t, void*);                     \
+    template int launch_batch_visc<UT, CT>(const StageArgs<UT, CT>&, const BatchArgs<UT, CT>&, int, int, void*); \
+    template int launch_batch_update<UT, CT, true>(const StageArgs<UT, CT>&, const BatchArgs<UT, CT>&, int, int, \
+                                                   void*);                                                   \
+    template int launch_batch_update<UT, CT, false>(const StageArgs<UT, CT>&, const BatchArgs<UT, CT>&, int, int, \
+                                                    void*);                                                  \
+    template int batch_occupancy<UT, CT>(int, int, bool, int*);
 OD_INST(float, float)
 OD_INST(double, float)
 #if !OD_FAST
@@ -966,6 +1209,12 @@ OD_INST(float, double)
 template int launch_sums<float>(const float*, double*, double*, int64_t, const double*, double, void*);
 template int launch_sums<double>(const double*, double*, double*, int64_t, const double*, double, void*);
 #endif
+
+int upload_ops(const Ops<float>& f32, const Ops<double>& f64) {
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_ops_f32), &f32, sizeof(f32)) != hipSuccess) return 1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_ops_f64), &f64, sizeof(f64)) != hipSuccess) return 1;
+    return 0;
+}
 
 #if OD_FAST
 }  // namespace fastmath

@@ -10,6 +10,8 @@ pt = i + 4j + 16k, elem = ex + nx*ey + nx*ny*ez (make_initial_state.py:16-22).
   fourier_state   periodic divergence-free random-Fourier perturbation of a
                   uniform flow (no reference counterpart; configs with
                   periodic boxes)
+  ensemble_boxes  sizes and Fourier parameters of an ensemble of periodic
+                  boxes (no reference counterpart)
 """
 import math
 
@@ -93,6 +95,28 @@ def fourier_state(nx, ny, nz, lx, ly, lz, u0=(0.1, 0.0, 0.0), amplitude=1e-2, n_
     r = np.full_like(X, rho)
     E = p / (GAMMA - 1.0) + 0.5 * r * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
     return np.concatenate([r, r * vel[0], r * vel[1], r * vel[2], E]).astype(dtype)
+
+
+def ensemble_boxes(n_boxes=24, seed=7, n_lo=16, n_hi=40, modes=(12, 32), amps=(0.02, 0.1),
+                   length=2.0 * math.pi):
+    """An ensemble of independent periodic cubes [0, length)^3: per box an
+    element count n ~ U{n_lo..n_hi} per side, a Fourier IC with n_modes ~
+    U{modes}, amplitude ~ U[amps) and its own seed, all drawn from one
+    generator seeded with `seed`.  Returns a list of dicts."""
+    rng = np.random.default_rng(seed)
+    n = rng.integers(n_lo, n_hi + 1, size=n_boxes)
+    m = rng.integers(modes[0], modes[1] + 1, size=n_boxes)
+    a = rng.uniform(amps[0], amps[1], size=n_boxes)
+    s = rng.integers(0, 2**31 - 1, size=n_boxes)
+    return [dict(nx=int(n[q]), ny=int(n[q]), nz=int(n[q]), lx=length, ly=length, lz=length, n_modes=int(m[q]),
+                 amplitude=float(a[q]), seed=int(s[q])) for q in range(n_boxes)]
+
+
+def ensemble_state(box, dtype=np.float32):
+    """Initial state of one ensemble box: Mach 0.1 mean flow in x plus its
+    divergence-free Fourier perturbation."""
+    return fourier_state(box["nx"], box["ny"], box["nz"], box["lx"], box["ly"], box["lz"], u0=(0.1, 0.0, 0.0),
+                         amplitude=box["amplitude"], n_modes=box["n_modes"], seed=box["seed"], dtype=dtype)
 
 
 def _fma32(a, b, c):

@@ -403,9 +403,13 @@ struct Solver {
     }
 
     int64_t eidx(int ex, int ey, int ez) const { return ex + (int64_t)nx * (ey + (int64_t)ny * ez); }
-    C mu_at(const C* s) const {
-        if (prm.viscosity_model == 1) return P::sutherland(s, mu_ref, t_ref, s_suth);
-        return mu_fixed;
+    // Per-element eddy viscosity added to the molecular one (no reference
+    // counterpart; empty = plain FR): mu = mu_mol + mu_t[e] in the compute type.
+    std::vector<C> mut;
+    C mu_at(const C* s, int64_t e) const {
+        C m = prm.viscosity_model == 1 ? P::sutherland(s, mu_ref, t_ref, s_suth) : mu_fixed;
+        if (!mut.empty()) m = m + mut[e];
+        return m;
     }
     static C pw(C t0, C t1, C t2, C t3) { return (t0 + t1) + (t2 + t3); }  // shfl butterfly
     static C seq0(C t0, C t1, C t2, C t3) {                              // acc = 0; acc += t
@@ -599,7 +603,7 @@ struct Solver {
                 P::euler(dir, st, fe);
                 if (prm.viscous) {
                     for (int c = 0; c < 15; ++c) gr[c] = g[c][pt];
-                    P::viscous(dir, st, gr, mu_at(st), fv);
+                    P::viscous(dir, st, gr, mu_at(st, e), fv);
                     for (int v = 0; v < 5; ++v) F[v][pt] = fe[v] - fv[v];
                 } else {
                     for (int v = 0; v < 5; ++v) F[v][pt] = fe[v];
@@ -634,8 +638,8 @@ struct Solver {
         C fi[5], fvl[5], fvr[5], fel[5], fer[5];
         P::riemann(prm.flux_scheme, ul, ur, n, fi);
         if (prm.viscous) {
-            P::viscous(dir, ul, gl_, mu_at(ul), fvl);
-            P::viscous(dir, ur, gr_, mu_at(ur), fvr);
+            P::viscous(dir, ul, gl_, mu_at(ul, el), fvl);
+            P::viscous(dir, ur, gr_, mu_at(ur, er), fvr);
         }
         P::euler(dir, ul, fel);
         P::euler(dir, ur, fer);
@@ -726,7 +730,7 @@ struct Solver {
                 }
                 if (prm.viscous) read_slice(1, &sl[upper ? 3 : 2][(er * 16 + fp) * 12], grd);
                 C fv[5] = {0, 0, 0, 0, 0}, fe[5];
-                if (prm.viscous) P::viscous(1, st, grd, mu_at(st), fv);
+                if (prm.viscous) P::viscous(1, st, grd, mu_at(st, er), fv);
                 P::euler(1, st, fe);
                 C v2r = st[2] * st[2] / st[0];
                 C wall[5] = {(C)0.0, (C)0.0, fe[2] - v2r, (C)0.0, (C)0.0};
@@ -841,8 +845,12 @@ void rk3_step(Solver<U, Q, R, C>& s, U* u, U* u1, double dt) {
 }
 
 template <class U, class Q, class R, class C>
-int run_step(const orc_params& p, void* state, double dt, int n) {
+int run_step(const orc_params& p, void* state, double dt, int n, const double* mu_t) {
     Solver<U, Q, R, C> s(p);
+    if (mu_t) {
+        s.mut.resize(s.E);
+        for (int64_t e = 0; e < s.E; ++e) s.mut[e] = (C)mu_t[e];
+    }
     U* u = (U*)state;
     std::vector<U> u1(5 * s.NP);
     for (int t = 0; t < n; ++t) rk3_step(s, u, u1.data(), dt);
@@ -991,7 +999,12 @@ void orc_fr_operators(double points[4], double d_matrix[16], double extrap_left[
 
 int orc_step(const orc_params* p, void* state, double dt, int n_steps) {
     if (!valid(p) || !state || n_steps < 0) return 1;
-    return OD_DISPATCH(run_step, *p, *p, state, dt, n_steps);
+    return OD_DISPATCH(run_step, *p, *p, state, dt, n_steps, (const double*)nullptr);
+}
+
+int orc_step_mut(const orc_params* p, void* state, const double* mu_t, double dt, int n_steps) {
+    if (!valid(p) || !state || !mu_t || n_steps < 0) return 1;
+    return OD_DISPATCH(run_step, *p, *p, state, dt, n_steps, mu_t);
 }
 
 int orc_rhs_total(const orc_params* p, const void* state, void* out) {

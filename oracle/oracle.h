@@ -57,6 +57,11 @@ void orc_fr_operators(double points[4], double d_matrix[16], double extrap_left[
  * size dt in place.  dt is the f32 config value widened to f64. */
 int orc_step(const orc_params* p, void* state, double dt, int n_steps);
 
+/* orc_step with a per-element eddy viscosity mu_t[E] (f64, converted to the
+ * compute type) added to the molecular viscosity at every viscous flux of the
+ * element: mu = mu_mol + mu_t[e].  No reference counterpart. */
+int orc_step_mut(const orc_params* p, void* state, const double* mu_t, double dt, int n_steps);
+
 /* Corrected RHS of one stage evaluated on `state`: RHS + sum of the FR face
  * corrections, exactly the value the RK blend consumes (compute_real type). */
 int orc_rhs_total(const orc_params* p, const void* state, void* out);

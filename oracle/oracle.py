@@ -46,6 +46,7 @@ def lib():
         L.orc_fr_operators.argtypes = [dp] * 7
         L.orc_fr_operators.restype = None
         L.orc_step.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.orc_step_mut.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_void_p, dp, ctypes.c_double, ctypes.c_int]
         L.orc_rhs_total.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_void_p, ctypes.c_void_p]
         L.orc_debug_array.argtypes = [ctypes.POINTER(OrcParams), ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.orc_point.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
@@ -104,6 +105,16 @@ def step(p, state, dt, n_steps):
     u = np.ascontiguousarray(state, dtype=_DT[p.u_real]).copy()
     assert u.size == 5 * n_points(p)
     _check(lib().orc_step(ctypes.byref(p), u.ctypes.data, float(np.float32(dt)), int(n_steps)), "step")
+    return u
+
+
+def step_mut(p, state, mu_t, dt, n_steps):
+    """step() with a per-element eddy viscosity mu_t[E] added to the molecular one."""
+    u = np.ascontiguousarray(state, dtype=_DT[p.u_real]).copy()
+    m = np.ascontiguousarray(mu_t, dtype=np.float64)
+    assert u.size == 5 * n_points(p) and m.size == p.nx * p.ny * p.nz
+    _check(lib().orc_step_mut(ctypes.byref(p), u.ctypes.data, m.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              float(np.float32(dt)), int(n_steps)), "step_mut")
     return u
 
 

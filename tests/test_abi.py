@@ -82,3 +82,16 @@ def test_host_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert sorted(host.EXPORTS) == sorted(names)
+
+
+def test_batch_create_rejects_bad_arguments_before_touching_a_device():
+    p = od.params(1, 1, 1, 1.0, 1.0, 1.0)
+    h = ctypes.c_void_p()
+    L = od.lib()
+    assert L.od_batch_create(ctypes.byref(p), (od.Box * 1)(), 0, 0, ctypes.byref(h)) == 1
+    assert b"at least one box" in L.od_last_error()
+    bad = (od.Box * 1)(od.Box(0, 2, 2, 1.0, 1.0, 1.0))
+    assert L.od_batch_create(ctypes.byref(p), bad, 1, 0, ctypes.byref(h)) == 1
+    bad = (od.Box * 1)(od.Box(2, 2, 2, 1.0, -1.0, 1.0))
+    assert L.od_batch_create(ctypes.byref(p), bad, 1, 0, ctypes.byref(h)) == 1
+    assert not h.value

@@ -1,0 +1,146 @@
+"""Ensembles (od_batch_*): every box of a batch advanced with one launch per
+kernel and stage must equal the same box run alone through od_step, byte for
+byte, for every schedule the autotuner can pick; tuning leaves the state
+unchanged.  Boxes have odd and even sizes in every direction so that tiles
+with a missing second element occur for each tile orientation."""
+import numpy as np
+import pytest
+
+import opendynamo_amd as od
+from helpers import smooth_state
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BOXES = [(3, 4, 5, 6.0, 2.0, 3.0), (4, 4, 4, 6.283185307179586, 6.283185307179586, 6.283185307179586),
+         (5, 3, 2, 4.0, 2.0, 2.5), (2, 5, 3, 3.0, 5.0, 3.0)]
+DT = [0.004, 0.003, 0.0025, 0.0035]
+
+SCHEDULES = [
+    dict(orient=0, pairs=1, order=0, groups_per_cu=0),
+    dict(orient=1, pairs=2, order=1, groups_per_cu=1),
+    dict(orient=2, pairs=1, order=1, groups_per_cu=2),
+    dict(orient=0, pairs=2, order=0, groups_per_cu=1),
+]
+
+VARIANTS = {
+    "f32_fast_ns_wall": dict(u_real=od.OD_F32, flux="slau2", y_wall=True, viscosity_model=1, mu_ref=0.002,
+                             fast_math=1),
+    "f32_strict_ns": dict(u_real=od.OD_F32, flux="ausm+", y_wall=False, mu_ref=0.002, fast_math=0),
+    "f64_ns_wall": dict(u_real=od.OD_F64, flux="slau2", y_wall=True, mu_ref=0.002),
+    "mixed_euler": dict(u_real=od.OD_F64, compute_real=od.OD_F32, q_real=od.OD_F32, rhs_real=od.OD_F32,
+                        flux="rusanov", viscous=False, body_force_x=0.0, rk_accum=1),
+}
+
+
+def announce(**case):
+    print("CASE", " ".join(f"{k}={v}" for k, v in case.items()), flush=True)
+
+
+def initial(q, dtype):
+    nx, ny, nz, lx, ly, lz = BOXES[q]
+    return smooth_state(nx, ny, nz, lx, ly, lz, seed=10 + q, amp=0.04, dtype=dtype)
+
+
+def single_runs(kw, n_steps):
+    out = []
+    for q, (nx, ny, nz, lx, ly, lz) in enumerate(BOXES):
+        p = od.params(nx, ny, nz, lx, ly, lz, **kw)
+        with od.Solver(p) as s:
+            s.set_state(initial(q, s.dtype))
+            s.step(DT[q], n_steps)
+            out.append(s.get_state())
+    return out
+
+
+def batch_params(kw):
+    return od.params(1, 1, 1, 1.0, 1.0, 1.0, **kw)
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_batch_equals_boxes_run_alone(variant):
+    kw = VARIANTS[variant]
+    n_steps = 3
+    ref = single_runs(kw, n_steps)
+    with od.Batch(batch_params(kw), BOXES) as b:
+        for sched in SCHEDULES:
+            announce(test="batch_equals_alone", variant=variant, **sched)
+            b.set_schedule(**sched)
+            for q in range(len(BOXES)):
+                b.set_state(q, initial(q, b.dtype))
+            b.step(DT, n_steps)
+            for q in range(len(BOXES)):
+                got = b.get_state(q)
+                assert got.tobytes() == ref[q].tobytes(), (variant, sched, q)
+
+
+def test_batch_f64_euler_matches_oracle():
+    """Transitively pinned above; checked directly once against the oracle."""
+    kw = dict(u_real=od.OD_F64, flux="rusanov", viscous=False, body_force_x=0.0)
+    boxes = BOXES[:2]
+    with od.Batch(batch_params(kw), boxes) as b:
+        b.set_schedule(orient=1, pairs=2, order=1, groups_per_cu=1)
+        for q in range(len(boxes)):
+            b.set_state(q, initial(q, np.float64))
+        b.step(DT[:2], 2)
+        for q, (nx, ny, nz, lx, ly, lz) in enumerate(boxes):
+            announce(test="batch_vs_oracle", box=q)
+            po = O.params(nx, ny, nz, float(np.float32(lx)), float(np.float32(ly)), float(np.float32(lz)),
+                          flux="rusanov", viscous=False, body_force_x=0.0)
+            want = O.step(po, initial(q, np.float64), float(np.float32(DT[q])), 2)
+            assert b.get_state(q).tobytes() == np.asarray(want, dtype=np.float64).tobytes()
+
+
+def test_autotune_keeps_state_and_result():
+    kw = VARIANTS["f32_fast_ns_wall"]
+    ref = single_runs(kw, 2)
+    with od.Batch(batch_params(kw), BOXES) as b:
+        for q in range(len(BOXES)):
+            b.set_state(q, initial(q, b.dtype))
+        before = [b.get_state(q).tobytes() for q in range(len(BOXES))]
+        announce(test="autotune")
+        best, best_ms, table = b.autotune(DT, steps_per_trial=1)
+        assert len(table) >= 24 and best_ms > 0
+        assert best == b.get_schedule()
+        assert min(t["ms_per_step"] for t in table) == best_ms
+        assert all(t["ms_per_step"] > 0 for t in table)
+        assert [b.get_state(q).tobytes() for q in range(len(BOXES))] == before
+        b.step(DT, 2)
+        for q in range(len(BOXES)):
+            assert b.get_state(q).tobytes() == ref[q].tobytes()
+
+
+def test_batch_sums_cfl_info_and_errors():
+    kw = VARIANTS["f64_ns_wall"]
+    with od.Batch(batch_params(kw), BOXES) as b:
+        with pytest.raises(od.ODError):
+            b.step(DT, 1)  # states not set
+        for q in range(len(BOXES)):
+            b.set_state(q, initial(q, np.float64))
+        with pytest.raises(od.ODError):
+            b.set_schedule(orient=3)
+        with pytest.raises(od.ODError):
+            b.set_state(0, np.zeros(7))
+        sums = b.sums()
+        dts = b.cfl_dt(0.3)
+        info = b.info()
+        assert info["boxes"] == len(BOXES) and info["launches_per_step"] == 6
+        assert info["dof"] == 64 * sum(x[0] * x[1] * x[2] for x in BOXES)
+    for q, (nx, ny, nz, lx, ly, lz) in enumerate(BOXES):
+        with od.Solver(od.params(nx, ny, nz, lx, ly, lz, **kw)) as s:
+            s.set_state(initial(q, np.float64))
+            assert s.sums().tobytes() == sums[q].tobytes()
+            assert s.cfl_dt(0.3) == dts[q]
+
+
+def test_per_box_launches_equal_batched():
+    kw = VARIANTS["f32_fast_ns_wall"]
+    ref = single_runs(kw, 2)
+    with od.Batch(batch_params(kw), BOXES) as b:
+        for q in range(len(BOXES)):
+            b.set_state(q, initial(q, b.dtype))
+        announce(test="per_box_timing")
+        ms = b.time_steps(DT, 2, per_box=True)
+        assert ms > 0
+        for q in range(len(BOXES)):
+            assert b.get_state(q).tobytes() == ref[q].tobytes()

@@ -1,0 +1,231 @@
+"""Learned closure on the GPU against the CPU oracle: the eddy-viscosity
+coupling (bitwise, strict builds), the feature kernel, the BF16 MFMA MLP
+forward and backward (tolerances stated per test), training, and both
+fallbacks (training range, kinetic-energy budget) reducing to the plain FR
+step exactly."""
+import numpy as np
+import pytest
+
+import opendynamo_amd as od
+from opendynamo_amd import ic
+from oracle import mlp as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2 * np.pi
+
+
+def announce(**case):
+    print("CASE", " ".join(f"{k}={v}" for k, v in case.items()), flush=True)
+
+
+def box_state(n=4, amp=0.05, seed=3, dtype=np.float64):
+    return ic.fourier_state(n, n, n, TWO_PI, TWO_PI, TWO_PI, u0=(0.05, 0.0, 0.0), amplitude=amp, n_modes=8,
+                            seed=seed, kmax=3, dtype=dtype)
+
+
+def ns_kw(real, fast=0):
+    return dict(u_real=real, flux="slau2", y_wall=False, viscous=True, viscosity_model=0, mu_ref=1e-3,
+                body_force_x=0.0, fast_math=fast)
+
+
+@pytest.mark.parametrize("real", [od.OD_F64, od.OD_F32])
+def test_eddy_viscosity_step_matches_oracle(real):
+    n = 4
+    announce(test="mu_t_step", real=real)
+    kw = ns_kw(real)
+    rng = np.random.default_rng(1)
+    mu_t = rng.uniform(0.0, 3e-3, n ** 3)
+    u0 = box_state(n, dtype=np.float64 if real == 8 else np.float32)
+    p = od.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **kw)
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        s.set_eddy_viscosity(mu_t)
+        s.step(0.01, 3)
+        got = s.get_state()
+    okw = dict(kw)
+    okw.pop("fast_math")
+    po = O.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **okw)
+    want = O.step_mut(po, u0, mu_t, 0.01, 3)
+    assert got.tobytes() == want.tobytes()
+    # a uniform mu_t is a shift of the molecular viscosity (f64: exact)
+    if real == od.OD_F64:
+        po2 = O.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **dict(okw, mu_ref=1e-3 + 2e-3))
+        a = O.step_mut(po, u0, np.full(n ** 3, 2e-3), 0.01, 2)
+        b = O.step(po2, u0, 0.01, 2)
+        assert a.tobytes() == b.tobytes()
+
+
+def test_zero_eddy_viscosity_is_plain_fr():
+    n = 4
+    p = od.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **ns_kw(od.OD_F32, fast=1))
+    u0 = box_state(n, dtype=np.float32)
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        s.step(0.01, 3)
+        plain = s.get_state()
+        s.set_state(u0)
+        s.set_eddy_viscosity(np.zeros(n ** 3))
+        s.step(0.01, 3)
+        assert s.get_state().tobytes() == plain.tobytes()
+        s.set_state(u0)
+        s.set_eddy_viscosity(None)
+        s.step(0.01, 3)
+        assert s.get_state().tobytes() == plain.tobytes()
+
+
+@pytest.mark.parametrize("real", [od.OD_F32, od.OD_F64])
+def test_features_match_oracle(real):
+    n = 4
+    announce(test="features", real=real)
+    u0 = box_state(n, amp=0.08, dtype=np.float64)
+    # density and pressure variations as well (the Fourier field has uniform rho and p)
+    X, Y, Z = ic.point_coords(n, n, n, TWO_PI, TWO_PI, TWO_PI)
+    f = 1.0 + 0.02 * np.sin(X + 2 * Z) * np.cos(Y)
+    u0 = u0.reshape(5, -1) * f
+    u0 = u0.ravel().astype(np.float64 if real == 8 else np.float32)
+    p = od.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **ns_kw(real))
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        got = s.features()
+    want = M.features(u0.astype(np.float64), n, n, n, float(np.float32(TWO_PI)), float(np.float32(TWO_PI)),
+                      float(np.float32(TWO_PI)), mu_mol=1e-3)
+    # f32 arithmetic on the GPU: 1e-3 of each feature's scale (features are
+    # O(1e-3..10); scales below 1e-4 count as 1e-4)
+    assert np.all(np.abs(want[:, 8:10]) > 1e-4)
+    scale = np.maximum(np.abs(want).max(0), 1e-4)
+    err = np.abs(got - want) / scale
+    assert err.max() < 1e-3, err.max(0)
+
+
+def mlp_case(seed=0, n=1000):
+    rng = np.random.default_rng(seed)
+    params = rng.normal(0.0, 0.25, M.NPARAM).astype(np.float32)
+    x = rng.normal(0.0, 1.0, (n, M.NF)).astype(np.float32)
+    shift = rng.normal(0.0, 0.1, M.NF).astype(np.float32)
+    scale = rng.uniform(0.8, 1.2, M.NF).astype(np.float32)
+    t = rng.uniform(0.0, 1.0, n).astype(np.float32)
+    return params, x, shift, scale, t
+
+
+def test_mlp_forward_matches_oracle():
+    params, x, shift, scale, _ = mlp_case(0, 1000)  # 1000 = 62 tiles of 16 + 8
+    announce(test="mlp_forward")
+    with od.MLP(params=params) as m:
+        m.set_scaling(shift, scale)
+        got = m.forward(x)
+        assert m.get_params().tobytes() == params.tobytes()
+    want, _ = M.forward(params, x, shift, scale, rounding=True)
+    # BF16 activations: an FP32-vs-exact sum difference can flip one BF16
+    # rounding; bound: 2 % of the output scale, median 1e-3
+    err = np.abs(got - want) / np.abs(want).max()
+    assert err.max() < 2e-2 and np.median(err) < 1e-3, (err.max(), np.median(err))
+    exact, _ = M.forward(params, x, shift, scale, rounding=False)
+    assert np.abs(got - exact).max() / np.abs(exact).max() < 5e-2
+
+
+def test_mlp_gradient_matches_oracle():
+    params, x, shift, scale, t = mlp_case(1, 777)
+    announce(test="mlp_grad")
+    with od.MLP(params=params) as m:
+        m.set_scaling(shift, scale)
+        loss, g = m.loss_grad(x, t)
+    wl, wg = M.loss_grad(params, x, t, shift, scale, rounding=True)
+    assert abs(loss - wl) / wl < 1e-2
+    rel = np.linalg.norm(g - wg) / np.linalg.norm(wg)
+    cos = float(np.dot(g, wg) / (np.linalg.norm(g) * np.linalg.norm(wg)))
+    assert rel < 3e-2 and cos > 0.999, (rel, cos)
+    # every block of the parameter vector, separately
+    for name, (o, sz) in M.OFF.items():
+        a, b = g[o:o + sz], wg[o:o + sz]
+        assert np.linalg.norm(a - b) <= 5e-2 * np.linalg.norm(b) + 1e-7, name
+
+
+def test_mlp_training_reduces_loss_and_is_deterministic():
+    rng = np.random.default_rng(5)
+    n = 20000
+    x = rng.normal(0.0, 1.0, (n, M.NF)).astype(np.float32)
+    t = (np.log1p(np.exp(0.8 * x[:, 0] - 0.5 * x[:, 3] + 0.3 * x[:, 7] * x[:, 2])) * 0.5).astype(np.float32)
+    announce(test="mlp_train")
+    with od.MLP() as a, od.MLP() as b:
+        p0 = a.get_params()
+        l0, _ = a.loss_grad(x, t)
+        ha = a.train(x, t, epochs=40, batch=4096, lr=3e-3, seed=11)
+        hb = b.train(x, t, epochs=40, batch=4096, lr=3e-3, seed=11)
+        la, _ = a.loss_grad(x, t)
+        assert a.get_params().tobytes() == b.get_params().tobytes()
+        assert ha.tobytes() == hb.tobytes()
+        assert not np.array_equal(a.get_params(), p0)
+    assert la < 0.1 * l0, (l0, la, ha[-5:])
+
+
+def closure_setup(n=4, fast=1):
+    p = od.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **ns_kw(od.OD_F32, fast=fast))
+    u0 = box_state(n, amp=0.08, dtype=np.float32)
+    params, _, _, _, _ = mlp_case(2, 16)
+    params = (params * 0.3).astype(np.float32)
+    return p, u0, params
+
+
+def test_closure_mu_t_matches_oracle_and_range_fallback():
+    p, u0, params = closure_setup()
+    announce(test="closure_mu_t")
+    with od.Solver(p) as s, od.MLP(params=params) as m:
+        s.set_state(u0)
+        feat = s.features()
+        lo, hi = feat[:, :12].min(0), feat[:, :12].max(0)
+        # a range that excludes the two elements with the largest feature 1
+        cut = np.sort(feat[:, 1])[-3]
+        hi2 = hi.copy()
+        hi2[1] = cut
+        m.set_scaling(np.zeros(12), np.ones(12), lo, hi2)
+        s.attach_closure(m, update_interval=2, budget_kappa=0.0)
+        s.step(0.01, 1)
+        mu = s.get_eddy_viscosity()
+        st = s.closure_stats()
+    want, inr = M.mu_t_from_features(params, feat.astype(np.float64), np.zeros(12), np.ones(12), lo, hi2)
+    assert st["updates"] == 1 and st["out_of_range"] == int((~inr).sum()) == 2
+    assert np.all(mu[~inr] == 0.0) and np.all(mu[inr] > 0.0)
+    assert np.abs(mu - want).max() <= 2e-2 * np.abs(want).max()
+
+
+def test_closure_all_out_of_range_is_plain_fr():
+    p, u0, params = closure_setup()
+    with od.Solver(p) as s, od.MLP(params=params) as m:
+        s.set_state(u0)
+        s.step(0.01, 4)
+        plain = s.get_state()
+        m.set_scaling(np.zeros(12), np.ones(12), np.full(12, 1e30), np.full(12, 2e30))
+        s.set_state(u0)
+        s.attach_closure(m, update_interval=2, budget_kappa=1.0)
+        s.step(0.01, 4)
+        assert s.get_state().tobytes() == plain.tobytes()
+        assert s.closure_stats()["out_of_range"] == 2 * 64
+
+
+def test_closure_budget_fallback_reverts_to_plain_fr():
+    """kappa tiny: the second update sees the model's dissipated energy exceed
+    kappa x the observed decay and switches the model off for the interval;
+    the run then equals a run with the first mu_t for one interval and the
+    plain step for the next, byte for byte."""
+    p, u0, params = closure_setup()
+    announce(test="closure_budget")
+    with od.Solver(p) as s, od.MLP(params=params) as m, od.Solver(p) as r:
+        m.set_scaling(np.zeros(12), np.ones(12))
+        s.set_state(u0)
+        s.attach_closure(m, update_interval=3, budget_kappa=1e-12)
+        s.step(0.01, 3)
+        mu1 = s.get_eddy_viscosity()
+        assert s.closure_stats()["active"] and mu1.max() > 0
+        s.step(0.01, 3)
+        st = s.closure_stats()
+        assert st["updates"] == 2 and st["budget_fallbacks"] == 1 and not st["active"]
+        assert np.all(s.get_eddy_viscosity() == 0.0)
+        got = s.get_state()
+        r.set_state(u0)
+        r.set_eddy_viscosity(mu1)
+        r.step(0.01, 3)
+        r.set_eddy_viscosity(None)
+        r.step(0.01, 3)
+        assert got.tobytes() == r.get_state().tobytes()
