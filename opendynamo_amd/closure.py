@@ -1,0 +1,360 @@
+"""Learned closure: training data from filtered fine runs, GPU training, and
+the spectral evaluation against the fine run (no reference counterpart).
+
+Pipeline (every step on the GPU through the library except the filter and
+the spectra, which are numpy/torch post-processing):
+
+  1. fine runs: plain FR P=3 NS on n_f^3 periodic boxes from a divergence-
+     free Fourier field, snapshots at fixed times;
+  2. filter: L2 projection of each fine snapshot onto the P=3 space of the
+     coarse mesh (n_c = n_f / r elements per side, r integer), applied to
+     rho, rho u_i and rho u_i u_j (Favre filtering);
+  3. target: per coarse element, the least-squares eddy viscosity of the
+     deviatoric subgrid stress tau_ij = bar(rho u_i u_j) - bar(rho u_i)
+     bar(rho u_j) / bar(rho) against -2 mu_t S~^d_ij, clipped at 0, divided by
+     rho_e Delta^2 |S|_e of the filtered state -> coefficient C >= 0;
+  4. features: the library's feature kernel on the filtered state (the same
+     kernel the closure evaluates at run time);
+  5. training: od_mlp_train (BF16 MFMA forward/backward, Adam) on the
+     standardised features, with the training range kept for the fallback;
+  6. evaluation: held-out seeds, energy spectra of coarse+closure, plain
+     coarse and plain intermediate runs against the fine run.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+import opendynamo_amd as od
+from opendynamo_amd import ic
+
+TWO_PI = 2.0 * math.pi
+
+
+# ---------------------------------------------------------------------------
+# FR P=3 point set
+# ---------------------------------------------------------------------------
+def _gl():
+    x, w = np.polynomial.legendre.leggauss(4)
+    return x, w
+
+
+def _legendre(a, x):
+    p0, p1 = np.ones_like(x), np.asarray(x, dtype=np.float64)
+    if a == 0:
+        return p0
+    for k in range(2, a + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    return p1
+
+
+def projection_matrix(r):
+    """[4][4r]: fine nodal values of the r fine elements along one axis of a
+    coarse element -> coarse nodal values of their L2 projection onto P3."""
+    x, w = _gl()
+    xc = np.array([-1.0 + (2 * s + 1 + x[i]) / r for s in range(r) for i in range(4)])
+    wc = np.array([w[i] / r for s in range(r) for i in range(4)])
+    P = np.zeros((4, 4 * r))
+    for a in range(4):
+        coef = (2 * a + 1) / 2.0 * wc * _legendre(a, xc)  # c_a = sum coef * f
+        P += np.outer(_legendre(a, x), coef)
+    return P
+
+
+def filter_field(f, nf, r):
+    """Project a fine nodal field [E_f * 64] (element-major, pt = i + 4j + 16k)
+    onto the coarse P3 space, n_c = nf / r; returns [E_c * 64]."""
+    nc = nf // r
+    P = projection_matrix(r)
+    a = np.asarray(f, dtype=np.float64).reshape(nc, r, nc, r, nc, r, 4, 4, 4)  # cz sz cy sy cx sx k j i
+    a = a.transpose(0, 2, 4, 1, 6, 3, 7, 5, 8).reshape(nc, nc, nc, 4 * r, 4 * r, 4 * r)
+    a = np.einsum("kz,abczyx->abckyx", P, a)
+    a = np.einsum("jy,abckyx->abckjx", P, a)
+    a = np.einsum("ix,abckjx->abckji", P, a)
+    return a.reshape(-1)
+
+
+def filter_state(state, nf, r):
+    """Filtered conservative state [5][E_c * 64] and the filtered products
+    bar(rho u_i u_j) [6][E_c * 64] (ij = xx, yy, zz, xy, xz, yz)."""
+    E = nf ** 3
+    u = np.asarray(state, dtype=np.float64).reshape(5, E * 64)
+    cons = np.stack([filter_field(u[v], nf, r) for v in range(5)])
+    pairs = [(1, 1), (2, 2), (3, 3), (1, 2), (1, 3), (2, 3)]
+    prod = np.stack([filter_field(u[a] * u[b] / u[0], nf, r) for a, b in pairs])
+    return cons, prod
+
+
+def _d_matrix():
+    x, _ = _gl()
+    D = np.zeros((4, 4))
+    for i in range(4):
+        for m in range(4):
+            if i == m:
+                D[i, m] = sum(1.0 / (x[i] - x[q]) for q in range(4) if q != i)
+            else:
+                num = np.prod([x[i] - x[q] for q in range(4) if q not in (i, m)])
+                den = np.prod([x[m] - x[q] for q in range(4) if q != m])
+                D[i, m] = num / den
+    return D
+
+
+def sgs_target(cons, prod, nc, L):
+    """mu_t per coarse element: argmin_mu sum_q w_q |tau^d + 2 mu S~^d|^2,
+    clipped at 0 (Favre-filtered velocity, density-weighted stress)."""
+    _, w = _gl()
+    D = _d_matrix()
+    E = nc ** 3
+    c = cons.reshape(5, E, 4, 4, 4)
+    rho = c[0]
+    ut = c[1:4] / rho  # Favre-filtered velocity
+    pr = prod.reshape(6, E, 4, 4, 4)
+    idx = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (0, 2): 4, (1, 2): 5}
+    tau = np.empty((3, 3, E, 4, 4, 4))
+    for a in range(3):
+        for b in range(3):
+            q = idx[(min(a, b), max(a, b))]
+            tau[a, b] = pr[q] - c[1 + a] * c[1 + b] / rho
+    tr = (tau[0, 0] + tau[1, 1] + tau[2, 2]) / 3.0
+    for a in range(3):
+        tau[a, a] -= tr
+    jac = 2.0 / (L / nc)
+    A = np.empty((3, 3) + ut.shape[1:])
+    for a in range(3):
+        A[a, 0] = np.einsum("im,ekjm->ekji", D, ut[a]) * jac
+        A[a, 1] = np.einsum("jm,ekmi->ekji", D, ut[a]) * jac
+        A[a, 2] = np.einsum("km,emji->ekji", D, ut[a]) * jac
+    div = A[0, 0] + A[1, 1] + A[2, 2]
+    S = 0.5 * (A + A.transpose(1, 0, 2, 3, 4, 5))
+    for a in range(3):
+        S[a, a] -= div / 3.0
+    W = np.einsum("k,j,i->kji", w, w, w)
+    num = -np.einsum("abekji,abekji,kji->e", tau, S, W)
+    den = 2.0 * np.einsum("abekji,abekji,kji->e", S, S, W)
+    return np.maximum(num / np.maximum(den, 1e-300), 0.0)
+
+
+# ---------------------------------------------------------------------------
+# Runs
+# ---------------------------------------------------------------------------
+def case_params(n, L, mu, flux="slau2", fast_math=1):
+    return od.params(n, n, n, L, L, L, u_real=od.OD_F32, flux=flux, y_wall=False, viscous=True,
+                     viscosity_model=0, mu_ref=mu, body_force_x=0.0, fast_math=fast_math)
+
+
+def initial_state(n, L, seed, n_modes, amplitude, kmax):
+    return ic.fourier_state(n, n, n, L, L, L, u0=(0.0, 0.0, 0.0), amplitude=amplitude, n_modes=n_modes, seed=seed,
+                            kmax=kmax, dtype=np.float32)
+
+
+def run(n, L, mu, seed, cfg, times, mlp=None, update_interval=5, budget_kappa=1.0, cfl=0.3):
+    """Advance from the IC, returning the state at each of `times` (ascending)
+    and the GPU wall time of the stepping.  dt from the CFL of the IC."""
+    p = case_params(n, L, mu, cfg.get("flux", "slau2"))
+    u0 = initial_state(n, L, seed, cfg["n_modes"], cfg["amplitude"], cfg["kmax"])
+    out, wall = [], 0.0
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        dt = s.cfl_dt(cfl)
+        if mlp is not None:
+            s.attach_closure(mlp, update_interval=update_interval, budget_kappa=budget_kappa)
+        t, step = 0.0, 0
+        for T in times:
+            n_steps = int(round((T - t) / dt))
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(dt, n_steps)
+            s.synchronize()
+            wall += time.perf_counter() - t0
+            t += n_steps * dt
+            step += n_steps
+            out.append(s.get_state())
+        stats = s.closure_stats() if mlp is not None else None
+    return dict(states=out, wall=wall, dt=dt, steps=step, stats=stats)
+
+
+# ---------------------------------------------------------------------------
+# Spectra
+# ---------------------------------------------------------------------------
+def uniform_velocity(state, n, m=4):
+    """Velocity on the uniform (m n)^3 grid of cell centres (m per element
+    per axis), by Lagrange interpolation inside each element."""
+    x, _ = _gl()
+    xu = -1.0 + (2.0 * np.arange(m) + 1.0) / m
+    I = np.ones((m, 4))
+    for a in range(m):
+        for b in range(4):
+            for q in range(4):
+                if q != b:
+                    I[a, b] *= (xu[a] - x[q]) / (x[b] - x[q])
+    u = np.asarray(state, dtype=np.float64).reshape(5, n, n, n, 4, 4, 4)  # ez ey ex k j i
+    vel = u[1:4] / u[0]
+    g = np.einsum("ak,bj,ci,vzyxkji->vzaybxc", I, I, I, vel)
+    return g.reshape(3, n * m, n * m, n * m)  # [v][z][y][x]
+
+
+def energy_spectrum(vel, L):
+    """Shell-summed E(k), k = 1 .. N/2 in units of 2 pi / L (integer shells)."""
+    N = vel.shape[1]
+    try:
+        import torch
+        if torch.cuda.is_available():
+            t = torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float32)).cuda()
+            uh = torch.fft.rfftn(t, dim=(1, 2, 3)) / (N ** 3)
+            e = (uh.abs() ** 2).sum(0).cpu().numpy().astype(np.float64)
+        else:
+            raise ImportError
+    except ImportError:
+        uh = np.fft.rfftn(vel, axes=(1, 2, 3)) / (N ** 3)
+        e = (np.abs(uh) ** 2).sum(0)
+    k1 = np.fft.fftfreq(N, 1.0 / N)
+    kz, ky = np.meshgrid(k1, k1, indexing="ij")
+    kx = np.arange(N // 2 + 1)
+    kk = np.sqrt(kz[:, :, None] ** 2 + ky[:, :, None] ** 2 + kx[None, None, :] ** 2)
+    # the rfft half-space counts the kx > 0 modes once: double them (except kx = 0 and Nyquist)
+    wgt = np.full(kx.shape, 2.0)
+    wgt[0] = 1.0
+    if N % 2 == 0:
+        wgt[-1] = 1.0
+    shell = np.rint(kk).astype(int)
+    E = np.bincount(shell.ravel(), weights=(0.5 * e * wgt[None, None, :]).ravel(), minlength=N)
+    return E[1:N // 2 + 1]
+
+
+def spectral_error(E, E_ref, k_max):
+    k = np.arange(1, k_max + 1)
+    return float(np.mean(np.abs(E[k - 1] - E_ref[k - 1]) / E_ref[k - 1]))
+
+
+# ---------------------------------------------------------------------------
+# Training set, training, evaluation
+# ---------------------------------------------------------------------------
+DEFAULT = dict(L=TWO_PI, mach=0.1, re=1600.0, n_modes=64, amplitude=0.1, kmax=4, flux="slau2", fine=48, ratio=3,
+               mid=32,
+               snap_times=[10.0, 20.0, 30.0, 40.0], eval_time=40.0, train_seeds=list(range(8)),
+               eval_seeds=[100, 101], epochs=300, batch=65536, lr=2e-3, update_interval=5, budget_kappa=1.0)
+
+
+def viscosity(cfg):
+    """Re = u_rms L_ref / nu with L_ref = 1, u_rms = Mach (c = 1), rho = 1."""
+    return cfg["mach"] * 1.0 / cfg["re"]
+
+
+def build_dataset(cfg, log=print):
+    mu = viscosity(cfg)
+    nf, r = cfg["fine"], cfg["ratio"]
+    nc = nf // r
+    X, T = [], []
+    pc = case_params(nc, cfg["L"], mu, cfg["flux"])
+    with od.Solver(pc) as coarse:
+        for seed in cfg["train_seeds"]:
+            res = run(nf, cfg["L"], mu, seed, cfg, cfg["snap_times"])
+            finite = [bool(np.isfinite(st).all()) for st in res["states"]]
+            log(f"fine seed {seed}: {res['steps']} steps, {res['wall']:.1f} s, finite snapshots {finite}")
+            for st, ok in zip(res["states"], finite):
+                if not ok:
+                    continue
+                cons, prod = filter_state(st, nf, r)
+                coarse.set_state(cons.reshape(-1).astype(np.float32))
+                f = coarse.features()
+                mu_t = sgs_target(cons, prod, nc, cfg["L"])
+                C = mu_t / (f[:, 12].astype(np.float64) * f[:, 13].astype(np.float64))
+                X.append(f[:, :12])
+                T.append(C.astype(np.float32))
+    return np.concatenate(X), np.concatenate(T)
+
+
+def train(cfg, X, T, device=0, log=print):
+    good = np.all(np.isfinite(X), axis=1) & np.isfinite(T)
+    if not good.any():
+        raise RuntimeError("no finite training samples (did the fine runs stay finite?)")
+    log(f"{int(good.sum())} of {len(T)} samples finite")
+    X, T = X[good], T[good]
+    T = np.minimum(T, np.quantile(T, 0.999))  # clip the extreme tail of the LSQ targets
+    shift = X.mean(0)
+    scale = 1.0 / np.maximum(X.std(0), 1e-6)
+    lo, hi = X.min(0), X.max(0)
+    m = od.MLP(device)
+    m.set_scaling(shift, scale, lo, hi)
+    t0 = time.perf_counter()
+    hist = m.train(X, T, epochs=cfg["epochs"], batch=cfg["batch"], lr=cfg["lr"], seed=1)
+    log(f"trained {cfg['epochs']} epochs on {len(T)} samples in {time.perf_counter() - t0:.1f} s; "
+        f"loss {hist[0]:.3e} -> {hist[-1]:.3e}; target variance {T.var():.3e}")
+    return m, dict(shift=shift, scale=scale, lo=lo, hi=hi, loss_history=hist, n_samples=int(len(T)),
+                   target_mean=float(T.mean()), target_var=float(T.var()))
+
+
+def evaluate(cfg, mlp, log=print):
+    mu = viscosity(cfg)
+    nf, r, nm = cfg["fine"], cfg["ratio"], cfg["mid"]
+    nc = nf // r
+    k_max = (nc * 4) // 3  # 2/3 of the coarse grid's Nyquist wavenumber (4 nc / 2)
+    rows = []
+    for seed in cfg["eval_seeds"]:
+        T = [cfg["eval_time"]]
+        fine = run(nf, cfg["L"], mu, seed, cfg, T)
+        E_ref = energy_spectrum(uniform_velocity(fine["states"][-1], nf), cfg["L"])
+        out = {"seed": seed, "fine_wall_s": fine["wall"]}
+        for name, n, net in (("coarse_closure", nc, mlp), ("coarse_plain", nc, None), ("mid_plain", nm, None)):
+            res = run(n, cfg["L"], mu, seed, cfg, T, mlp=net, update_interval=cfg["update_interval"],
+                      budget_kappa=cfg["budget_kappa"])
+            E = energy_spectrum(uniform_velocity(res["states"][-1], n), cfg["L"])
+            out[name] = dict(err=spectral_error(E, E_ref, k_max), wall_s=res["wall"], steps=res["steps"],
+                             spectrum=E[:k_max].tolist(), stats=res["stats"])
+            log(f"seed {seed} {name}: spectral error {out[name]['err']:.4f}, {res['wall']:.2f} s")
+        out["fine_spectrum"] = E_ref[:k_max].tolist()
+        rows.append(out)
+    return dict(k_max=k_max, runs=rows)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", default="closure_run")
+    ap.add_argument("--fine", type=int, default=DEFAULT["fine"])
+    ap.add_argument("--ratio", type=int, default=DEFAULT["ratio"])
+    ap.add_argument("--mid", type=int, default=DEFAULT["mid"])
+    ap.add_argument("--seeds", type=int, default=len(DEFAULT["train_seeds"]))
+    ap.add_argument("--epochs", type=int, default=DEFAULT["epochs"])
+    ap.add_argument("--snap-times", type=float, nargs="+", default=DEFAULT["snap_times"])
+    ap.add_argument("--eval-time", type=float, default=DEFAULT["eval_time"])
+    ap.add_argument("--weights", default=None, help="skip training: load weights (.npz) and evaluate")
+    ap.add_argument("--dataset", default=None, help="skip the fine runs: load features/targets (.npz)")
+    ap.add_argument("--flux", default=DEFAULT["flux"])
+    ap.add_argument("--kmax", type=int, default=DEFAULT["kmax"])
+    args = ap.parse_args(argv)
+    cfg = dict(DEFAULT, fine=args.fine, ratio=args.ratio, mid=args.mid, train_seeds=list(range(args.seeds)),
+               epochs=args.epochs, snap_times=args.snap_times, eval_time=args.eval_time, flux=args.flux,
+               kmax=args.kmax)
+    os.makedirs(args.out, exist_ok=True)
+    log = lambda s: print(s, flush=True)
+    report = {"config": cfg}
+    if args.weights:
+        w = np.load(args.weights)
+        mlp = od.MLP(params=w["params"])
+        mlp.set_scaling(w["shift"], w["scale"], w["lo"], w["hi"])
+    else:
+        if args.dataset:
+            d = np.load(args.dataset)
+            X, T = d["X"], d["T"]
+        else:
+            t0 = time.perf_counter()
+            X, T = build_dataset(cfg, log)
+            report["dataset_s"] = time.perf_counter() - t0
+            np.savez(os.path.join(args.out, "closure_dataset.npz"), X=X, T=T)
+        mlp, info = train(cfg, X, T, log=log)
+        np.savez(os.path.join(args.out, "closure_weights.npz"), params=mlp.get_params(), shift=info["shift"],
+                 scale=info["scale"], lo=info["lo"], hi=info["hi"])
+        report["training"] = {k: (v.tolist() if isinstance(v, np.ndarray) else v) for k, v in info.items()}
+    report["evaluation"] = evaluate(cfg, mlp, log)
+    with open(os.path.join(args.out, "closure_report.json"), "w") as f:
+        json.dump(report, f)
+    log(json.dumps({k: report["evaluation"]["runs"][0][k]["err"] for k in ("coarse_closure", "coarse_plain",
+                                                                           "mid_plain")}))
+
+
+if __name__ == "__main__":
+    main()

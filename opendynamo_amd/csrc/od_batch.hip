@@ -168,6 +168,13 @@ int batch_launch_t(od_batch* b, int kind, int stage) {
     ba.orient = s.orient;
     ba.order = s.order;
     ba.stage = stage;
+    if (b->boxes.size() <= (size_t)od::kInlineBoxes) {
+        int64_t t = 0;
+        for (size_t i = 0; i < b->boxes.size(); ++i) {
+            ba.tile_off[i] = (int32_t)t;
+            t += box_tiles(b->boxes[i], s.orient);
+        }
+    }
     int64_t g = ba.n_units;
     if (s.groups_per_cu > 0) {
         const int k = std::min(s.groups_per_cu, occupancy_t<UT, CT>(b, kind, s.pairs));

@@ -66,6 +66,8 @@ struct BoxDev {
     UT uc[3];          // and in the state type
 };
 
+constexpr int kInlineBoxes = 64;  // batches up to this size search the tile offsets in kernel arguments
+
 template <class UT, class CT>
 struct BatchArgs {
     const BoxDev<UT, CT>* boxes;  // device array, sorted by tile_off
@@ -73,6 +75,7 @@ struct BatchArgs {
     int32_t orient;               // element pair direction of a tile: 0 x, 1 y, 2 z
     int32_t order;                // 0 round robin, 1 contiguous range per XCD
     int32_t stage;                // 0..2
+    int32_t tile_off[kInlineBoxes];  // first tile of each box (n_boxes <= kInlineBoxes)
 };
 
 // Kernel launchers (explicitly instantiated for UT, CT in {float, double}).

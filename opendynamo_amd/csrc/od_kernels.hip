@@ -979,12 +979,22 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
     const bool ok = t < b.n_tiles;
     const int tt = ok ? t : 0;
     int lo = 0, hi = b.n_boxes - 1;
-    while (lo < hi) {  // last box whose first tile is <= tt
-        const int mid = (lo + hi + 1) >> 1;
-        if (uni(b.boxes[mid].tile_off) <= tt)
-            lo = mid;
-        else
-            hi = mid - 1;
+    if (b.n_boxes <= kInlineBoxes) {  // kernel-argument copy: scalar loads, no memory latency chain
+        while (lo < hi) {  // last box whose first tile is <= tt
+            const int mid = (lo + hi + 1) >> 1;
+            if (b.tile_off[mid] <= tt)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+    } else {
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (uni(b.boxes[mid].tile_off) <= tt)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
     }
     const BoxDev<UT, CT>& bx = b.boxes[lo];
     ab = a;
@@ -1025,6 +1035,13 @@ __global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT
     __shared__ ViscLds<CT> L[PAIRS];
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
+    if ((int)gridDim.x >= b.n_units && b.order == 0) {  // one unit per work-group: no loop
+        StageArgs<UT, CT> ab;
+        Tile tl;
+        batch_tile(a, b, (int)blockIdx.x * PAIRS + pair, ab, tl);
+        visc_body(ab, tl, L[pair], wave, pt);
+        return;
+    }
     for (int it = 0;; ++it) {
         const int unit = sched_unit(b.order, b.n_units, it);
         if (unit < 0) break;
@@ -1046,6 +1063,13 @@ __global__ __launch_bounds__(128 * PAIRS) void od_batch_update_kernel(StageArgs<
     __shared__ UpdateLds<CT, VISC> L[PAIRS];
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
+    if ((int)gridDim.x >= b.n_units && b.order == 0) {  // one unit per work-group: no loop
+        StageArgs<UT, CT> ab;
+        Tile tl;
+        batch_tile(a, b, (int)blockIdx.x * PAIRS + pair, ab, tl);
+        update_body<UT, CT, VISC>(ab, tl, L[pair], wave, pt);
+        return;
+    }
     for (int it = 0;; ++it) {
         const int unit = sched_unit(b.order, b.n_units, it);
         if (unit < 0) break;
