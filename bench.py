@@ -92,10 +92,10 @@ def traffic_from_profile():
     """HBM bytes per od_update_kernel launch measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), if the
     committed summary exists."""
-    path = os.path.join(ROOT, "profiles", "od_update_kernel_48cubed_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "fr_step_kernels_48cubed_f32_pmc.txt")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"traffic_bytes_per_launch_fast:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 

@@ -188,7 +188,8 @@ od_status od_batch_info(od_batch* b, double out[4]);
  * element's resolved field (features below) through 3 hidden layers of 64
  * GELU units to a coefficient C = softplus(.) >= 0; the element's eddy
  * viscosity mu_t = rho_e * Delta^2 * |S|_e * C is added to the molecular
- * viscosity of every viscous flux of the element (mu = mu_mol + mu_t[e]).
+ * viscosity of every viscous flux of the element (mu = mu_mol + mu_t[e]);
+ * C = s * softplus(.) with the output scale s of od_mlp_set_output_scale.
  * Weights and activations are BF16 on MFMA with FP32 accumulation.
  * Features (element means with GL weights, Delta = element size / (P+1),
  * |S| = sqrt(2 S^d:S^d)):
@@ -208,10 +209,10 @@ typedef struct od_mlp od_mlp;
 
 typedef struct od_closure_opts {
     int32_t update_interval;   /* steps between eddy-viscosity updates (>= 1) */
-    double budget_kappa;       /* kinetic-energy budget check: when the energy the model
-                                  dissipated over the last interval exceeds kappa times the
-                                  observed kinetic-energy decay, or the kinetic energy grew,
-                                  the next interval runs the plain FR step; <= 0 disables it */
+    double budget_kappa;       /* kinetic-energy budget check: when, over an interval in
+                                  which the kinetic energy decayed, the energy the model
+                                  dissipated exceeds kappa times the observed decay, the next
+                                  interval runs the plain FR step; <= 0 disables it */
 } od_closure_opts;
 
 od_status od_mlp_create(int device, od_mlp** out);
@@ -223,6 +224,9 @@ od_status od_mlp_get_params(od_mlp* m, float* params, size_t n);
  * mu_t = 0 (the plain FR step). */
 od_status od_mlp_set_scaling(od_mlp* m, const float shift[12], const float scale[12], const float lo[12],
                              const float hi[12]);
+/* Output scale s: C = s * softplus(z) (default 1), so the network trains on
+ * targets of order one. */
+od_status od_mlp_set_output_scale(od_mlp* m, float s);
 /* Forward pass on raw features x[n][12] -> C[n] (no range check). */
 od_status od_mlp_forward(od_mlp* m, const float* x, size_t n, float* y);
 /* Mean squared error of C against t[n] and its gradient (OD_MLP_PARAMS). */

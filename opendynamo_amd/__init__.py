@@ -27,6 +27,7 @@ EXPORTS = [
     "od_batch_set_schedule", "od_batch_get_schedule", "od_batch_autotune", "od_batch_time_steps",
     "od_batch_info",
     "od_mlp_create", "od_mlp_destroy", "od_mlp_set_params", "od_mlp_get_params", "od_mlp_set_scaling",
+    "od_mlp_set_output_scale",
     "od_mlp_forward", "od_mlp_loss_grad", "od_mlp_train", "od_closure_features", "od_closure_attach",
     "od_set_eddy_viscosity", "od_get_eddy_viscosity", "od_closure_stats",
 ]
@@ -132,6 +133,7 @@ def lib():
         L.od_mlp_set_params.argtypes = [vp, fp, ctypes.c_size_t]
         L.od_mlp_get_params.argtypes = [vp, fp, ctypes.c_size_t]
         L.od_mlp_set_scaling.argtypes = [vp, fp, fp, fp, fp]
+        L.od_mlp_set_output_scale.argtypes = [vp, ctypes.c_float]
         L.od_mlp_forward.argtypes = [vp, fp, ctypes.c_size_t, fp]
         L.od_mlp_loss_grad.argtypes = [vp, fp, fp, ctypes.c_size_t, dp, fp]
         L.od_mlp_train.argtypes = [vp, fp, fp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
@@ -480,6 +482,10 @@ class MLP:
         sh, sc, l, h = f(shift, 0.0), f(scale, 1.0), f(lo, -np.inf), f(hi, np.inf)
         self._scaling = (sh, sc, l, h)
         _check(lib().od_mlp_set_scaling(self._h, _fp(sh), _fp(sc), _fp(l), _fp(h)), "od_mlp_set_scaling")
+
+    def set_output_scale(self, s):
+        """C = s * softplus(z)."""
+        _check(lib().od_mlp_set_output_scale(self._h, float(s)), "od_mlp_set_output_scale")
 
     def forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_FEATURES)

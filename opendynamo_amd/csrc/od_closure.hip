@@ -254,6 +254,7 @@ struct DevNet {
 
 struct Scaling {
     float shift[NF], scale[NF], lo[NF], hi[NF];
+    float out_scale;  // C = out_scale * softplus(z)
 };
 
 __global__ void od_mlp_cast_kernel(const float* __restrict__ p, unsigned short* w1, unsigned short* w2,
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(256) void od_mlp_infer_kernel(DevNet net, Scaling s
                 bool in = true;
 #pragma unroll
                 for (int c = 0; c < NF; ++c) in = in && fe[c] >= sc.lo[c] && fe[c] <= sc.hi[c];
-                const float m = in ? fe[12] * fe[13] * softplus(z4[q]) : 0.0f;
+                const float m = in ? fe[12] * fe[13] * (sc.out_scale * softplus(z4[q])) : 0.0f;
                 mu_t[e] = (CT)m;
                 eps[e] = m * fe[15];
                 out_of_range[e] = in ? 0 : 1;
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(256) void od_mlp_forward_kernel(DevNet net, Scaling
         if (r == 0)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (e0 + 4 * g + q < n) y[e0 + 4 * g + q] = softplus(z4[q]);
+                if (e0 + 4 * g + q < n) y[e0 + 4 * g + q] = sc.out_scale * softplus(z4[q]);
     }
 }
 
@@ -549,9 +550,9 @@ __global__ __launch_bounds__(128) void od_mlp_grad_kernel(DevNet net, Scaling sc
             float d = 0.0f;
             if (e < n) {
                 const int64_t src = idx ? (int64_t)idx[e] : e;
-                const float y = softplus(z4[q]), err = y - tgt[src];
+                const float y = sc.out_scale * softplus(z4[q]), err = y - tgt[src];
                 if (r == 0) sse += err * err;
-                d = 2.0f * err * inv_n * sigmoid(z4[q]);
+                d = 2.0f * err * inv_n * sc.out_scale * sigmoid(z4[q]);
             }
             dz4[q] = d;
         }
@@ -856,9 +857,13 @@ od_status od_closure_update_internal(od_ctx* c, double dt_interval) {
     // budget check over the interval that just ended
     bool active = true;
     if (c->closure.budget_kappa > 0.0 && c->ke_prev >= 0.0 && c->closure_active) {
+        // The model over-dissipates when, over an interval in which the kinetic
+        // energy decayed, the energy it removed exceeds kappa times the whole
+        // observed decay.  An interval in which the energy grew is not held
+        // against the model (it only removes energy).
         const double dke = c->ke_prev - ke;               // observed decay
         const double work = c->eps_prev * dt_interval;    // energy the model removed
-        if (dke < 0.0 || work > c->closure.budget_kappa * dke) active = false;
+        if (dke > 0.0 && work > c->closure.budget_kappa * dke) active = false;
     }
     if (!active) {
         const unsigned fb = (unsigned)((c->E + 255) / 256);
@@ -931,6 +936,7 @@ od_status od_mlp_create(int device, od_mlp** out) {
         m->sc.lo[q] = -INFINITY;
         m->sc.hi[q] = INFINITY;
     }
+    m->sc.out_scale = 1.0f;
     od_status s = od_mlp_set_params(m, p.data(), p.size());
     if (s != OD_OK) {
         od_mlp_destroy(m);
@@ -983,6 +989,13 @@ od_status od_mlp_set_scaling(od_mlp* m, const float shift[12], const float scale
         m->sc.lo[q] = lo[q];
         m->sc.hi[q] = hi[q];
     }
+    return OD_OK;
+}
+
+od_status od_mlp_set_output_scale(od_mlp* m, float s) {
+    if (!m) return fail(OD_ERR_INVALID, "od_mlp_set_output_scale: null argument");
+    if (!(s > 0.0f) || !std::isfinite(s)) return fail(OD_ERR_INVALID, "od_mlp_set_output_scale: scale must be positive");
+    m->sc.out_scale = s;
     return OD_OK;
 }
 

@@ -735,13 +735,27 @@ struct ViscLds {
     CT sJmp[kEpb][6][5][16];
 };
 
+// With the prefetch (f32 compute) the face-correction array sJ shares the
+// prefetched neighbour traces (group A has sJ's exact layout and each face
+// point's lane reads its entries there before it writes them), and the
+// volume-flux staging sF of element w shares the BR1 jumps of element w
+// (read only by wave w, before it writes sF).
 template <class CT, bool VISC>
 struct UpdateLds {
-    float pf[std::is_same<CT, float>::value ? (VISC ? kPfFloats : kPfC) : 4];
+    static constexpr bool kPF = std::is_same<CT, float>::value;
+    float pf[kPF ? (VISC ? kPfFloats : kPfC) : 4];
     CT sU[kEpb][5][64];
-    CT sF[kEpb][5][64];
-    CT sJ[kEpb][6][5][16];
     CT sJmp[kEpb][6][5][16];
+    CT sJ_own[kPF ? 0 : kEpb][6][5][16];  // zero-length when aliased
+    CT sF_own[VISC ? 0 : kEpb][5][64];
+    __device__ CT (*sJ())[6][5][16] {
+        if constexpr (kPF) return reinterpret_cast<CT(*)[6][5][16]>(pf + kPfA);
+        else return sJ_own;
+    }
+    __device__ CT (*sF(int w))[64] {
+        if constexpr (VISC) return reinterpret_cast<CT(*)[64]>(&sJmp[w][0][0][0]);
+        else return sF_own[w];
+    }
 };
 
 // Face traces of the state -> tr_out (wave w loads element w of the tile).
@@ -803,11 +817,11 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     const Ops<CT>& o = dev_ops<CT>();
     if constexpr (PF) prefetch_issue(a, tl, L.pf, VISC ? kPfChunksFull : kPfChunksEuler, wave, pt);
 
-    UT ucur[5], uprv[5];
+    constexpr bool SAME = std::is_same<UT, CT>::value;  // sU holds the state exactly
+    UT ucur[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         ucur[v] = mine ? a.uin[v * NP + gbase] : (UT)1;
-        uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
         L.sU[wave][v][pt] = (CT)ucur[v];
     }
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
@@ -843,7 +857,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
         }
         if (d > 0) __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 5; ++v) L.sF[wave][v][pt] = F[v];
+        for (int v = 0; v < 5; ++v) L.sF(wave)[v][pt] = F[v];
         __syncthreads();
         const int idx = d == 0 ? i : (d == 1 ? j : k);
         const int base = d == 0 ? 4 * j + 16 * k : (d == 1 ? i + 16 * k : i + 4 * j);
@@ -856,24 +870,25 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
         for (int v = 0; v < 5; ++v) {
             CT dd = 0;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) dd += Dd[m] * L.sF[wave][v][base + m * stride];
+            for (int m = 0; m < 4; ++m) dd += Dd[m] * L.sF(wave)[v][base + m * stride];
             rs[v] += dd * jac;
         }
     }
-    CT r[5];
+    // the volume RHS waits in LDS (this wave's sF slot) during the face passes
+    __syncthreads();
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         CT val = -rs[v];
         if (v == 1) val += a.bf;
-        r[v] = a.round_rhs ? (CT)(float)val : val;  // RHS_REAL round trip (rhs.cu:509, rk.cu:52)
+        L.sF(wave)[v][pt] = a.round_rhs ? (CT)(float)val : val;  // RHS_REAL round trip (rhs.cu:509, rk.cu:52)
     }
 
     // Interface fluxes at all own faces, direction by direction.
     if (wave == 0) {
-        flux_pass<0, VISC, PF>(a, L.sU, L.sJ, tl, pfc, pt);
-        flux_pass<2, VISC, PF>(a, L.sU, L.sJ, tl, pfc, pt);
+        flux_pass<0, VISC, PF>(a, L.sU, L.sJ(), tl, pfc, pt);
+        flux_pass<2, VISC, PF>(a, L.sU, L.sJ(), tl, pfc, pt);
     } else {
-        flux_pass<1, VISC, PF>(a, L.sU, L.sJ, tl, pfc, pt);
+        flux_pass<1, VISC, PF>(a, L.sU, L.sJ(), tl, pfc, pt);
     }
     __syncthreads();
 
@@ -882,26 +897,34 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     const CT gLj = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], j), gRj = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], j);
     const CT gLk = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], k), gRk = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], k);
     const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
+    UT uprv[5], ucr[5];
+    CT r[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
+        ucr[v] = SAME ? (UT)L.sU[wave][v][pt] : ucur[v];
+        r[v] = L.sF(wave)[v][pt];
+    }
     UT unew[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        const CT(*J)[5][16] = L.sJ[wave];
+        const CT(*J)[5][16] = L.sJ()[wave];
         CT cx = -(J[0][v][fpx] * gLi * a.jx) - (J[1][v][fpx] * gRi * a.jx);
         CT cy = -(J[2][v][fpy] * gLj * a.jy) - (J[3][v][fpy] * gRj * a.jy);
         CT cz = -(J[4][v][fpz] * gLk * a.jz) - (J[5][v][fpz] * gRk * a.jz);
         CT rt = r[v] + (cx + cy + cz);
         if (a.mode == 1) {
             if (mine) a.dbg[v * NP + gbase] = rt;
-            unew[v] = ucur[v];
+            unew[v] = ucr[v];
             continue;
         }
         if (!a.accum_state) {
-            CT up = (CT)uprv[v], uc = (CT)ucur[v];
+            CT up = (CT)uprv[v], uc = (CT)ucr[v];
             CT val = a.stage1 ? (a.ca * up + a.cc * rt) : (a.ca * up + a.cb * uc + a.cc * rt);
             unew[v] = (UT)val;
         } else {
             UT rr = (UT)rt;
-            unew[v] = a.stage1 ? (UT)(a.ua * uprv[v] + a.uc * rr) : (UT)(a.ua * uprv[v] + a.ub * ucur[v] + a.uc * rr);
+            unew[v] = a.stage1 ? (UT)(a.ua * uprv[v] + a.uc * rr) : (UT)(a.ua * uprv[v] + a.ub * ucr[v] + a.uc * rr);
         }
         if (mine) a.uout[v * NP + gbase] = unew[v];
     }
@@ -928,7 +951,7 @@ __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
 }
 
 template <class UT, class CT, bool VISC>
-__global__ __launch_bounds__(128) void od_update_kernel(StageArgs<UT, CT> a) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void od_update_kernel(StageArgs<UT, CT> a) {
     __shared__ UpdateLds<CT, VISC> L;
     update_body<UT, CT, VISC>(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }

@@ -78,8 +78,8 @@ def _round(x, on):
     return bf16(np.asarray(x, dtype=np.float32)) if on else np.asarray(x, dtype=np.float64)
 
 
-def forward(params, x, shift=None, scale=None, rounding=True):
-    """C = softplus(MLP(standardised x)); also returns the cache for loss_grad."""
+def forward(params, x, shift=None, scale=None, rounding=True, out_scale=1.0):
+    """C = out_scale * softplus(MLP(standardised x)); also returns the cache for loss_grad."""
     P = unpack(params)
     x = np.asarray(x, dtype=np.float64).reshape(-1, NF)
     shift = np.zeros(NF) if shift is None else np.asarray(shift, dtype=np.float64)
@@ -94,18 +94,18 @@ def forward(params, x, shift=None, scale=None, rounding=True):
     z3 = h2 @ W3.T + P["b3"]
     h3 = _round(gelu(z3), rounding)
     z4 = h3 @ w4 + P["b4"]
-    y = softplus(z4)
+    y = float(np.float32(out_scale)) * softplus(z4)
     return y, dict(h0=h0, z1=z1, h1=h1, z2=z2, h2=h2, z3=z3, h3=h3, z4=z4, W2=W2, W3=W3, w4=w4)
 
 
-def loss_grad(params, x, t, shift=None, scale=None, rounding=True):
+def loss_grad(params, x, t, shift=None, scale=None, rounding=True, out_scale=1.0):
     """Mean squared error of C against t and its gradient (NPARAM)."""
-    y, c = forward(params, x, shift, scale, rounding)
+    y, c = forward(params, x, shift, scale, rounding, out_scale)
     t = np.asarray(t, dtype=np.float64)
     n = y.size
     err = y - t
     loss = float(np.mean(err * err))
-    dz4 = 2.0 * err / n * sigmoid(c["z4"])
+    dz4 = 2.0 * err / n * float(np.float32(out_scale)) * sigmoid(c["z4"])
     g = {"b4": np.sum(dz4), "w4": c["h3"].T @ dz4}
     dz3 = np.outer(dz4, c["w4"]) * gelu_d(c["z3"])
     g["W3"], g["b3"] = dz3.T @ c["h2"], dz3.sum(0)
@@ -218,8 +218,8 @@ def features(state, nx, ny, nz, lx, ly, lz, mu_mol):
     return f
 
 
-def mu_t_from_features(params, feat, shift, scale, lo, hi, rounding=True):
+def mu_t_from_features(params, feat, shift, scale, lo, hi, rounding=True, out_scale=1.0):
     """Eddy viscosity per element as the inference kernel forms it."""
-    y, _ = forward(params, feat[:, :NF], shift, scale, rounding)
+    y, _ = forward(params, feat[:, :NF], shift, scale, rounding, out_scale)
     inr = np.all((feat[:, :NF] >= lo) & (feat[:, :NF] <= hi), axis=1)
     return np.where(inr, feat[:, 12] * feat[:, 13] * y, 0.0), inr

@@ -125,13 +125,19 @@ def test_mlp_forward_matches_oracle():
     assert np.abs(got - exact).max() / np.abs(exact).max() < 5e-2
 
 
-def test_mlp_gradient_matches_oracle():
+@pytest.mark.parametrize("out_scale", [1.0, 0.01])
+def test_mlp_gradient_matches_oracle(out_scale):
     params, x, shift, scale, t = mlp_case(1, 777)
-    announce(test="mlp_grad")
+    t = (t * out_scale).astype(np.float32)
+    announce(test="mlp_grad", out_scale=out_scale)
     with od.MLP(params=params) as m:
         m.set_scaling(shift, scale)
+        m.set_output_scale(out_scale)
         loss, g = m.loss_grad(x, t)
-    wl, wg = M.loss_grad(params, x, t, shift, scale, rounding=True)
+        y = m.forward(x)
+    wl, wg = M.loss_grad(params, x, t, shift, scale, rounding=True, out_scale=out_scale)
+    wy, _ = M.forward(params, x, shift, scale, rounding=True, out_scale=out_scale)
+    assert np.abs(y - wy).max() <= 2e-2 * np.abs(wy).max()
     assert abs(loss - wl) / wl < 1e-2
     rel = np.linalg.norm(g - wg) / np.linalg.norm(wg)
     cos = float(np.dot(g, wg) / (np.linalg.norm(g) * np.linalg.norm(wg)))
@@ -206,9 +212,9 @@ def test_closure_all_out_of_range_is_plain_fr():
 
 def test_closure_budget_fallback_reverts_to_plain_fr():
     """kappa tiny: the second update sees the model's dissipated energy exceed
-    kappa x the observed decay and switches the model off for the interval;
-    the run then equals a run with the first mu_t for one interval and the
-    plain step for the next, byte for byte."""
+    kappa x the observed decay (the viscous box decays) and switches the model
+    off for the interval; the run then equals a run with the first mu_t for
+    one interval and the plain step for the next, byte for byte."""
     p, u0, params = closure_setup()
     announce(test="closure_budget")
     with od.Solver(p) as s, od.MLP(params=params) as m, od.Solver(p) as r:
@@ -217,9 +223,11 @@ def test_closure_budget_fallback_reverts_to_plain_fr():
         s.attach_closure(m, update_interval=3, budget_kappa=1e-12)
         s.step(0.01, 3)
         mu1 = s.get_eddy_viscosity()
+        ke1 = s.closure_stats()["kinetic_energy"]
         assert s.closure_stats()["active"] and mu1.max() > 0
         s.step(0.01, 3)
         st = s.closure_stats()
+        assert st["kinetic_energy"] < ke1  # the interval decayed
         assert st["updates"] == 2 and st["budget_fallbacks"] == 1 and not st["active"]
         assert np.all(s.get_eddy_viscosity() == 0.0)
         got = s.get_state()
