@@ -237,3 +237,30 @@ def test_closure_budget_fallback_reverts_to_plain_fr():
         r.set_eddy_viscosity(None)
         r.step(0.01, 3)
         assert got.tobytes() == r.get_state().tobytes()
+
+
+def test_committed_closure_model_runs_on_the_coarse_mesh():
+    """The trained network shipped with the package (48^3 -> 16^3, Re 400)
+    loads, attaches and produces a positive eddy viscosity on its own coarse
+    workload; the first 50 steps stay finite."""
+    import os
+    from opendynamo_amd import closure as C
+    w = np.load(os.path.join(os.path.dirname(od.__file__), "closure_model", "closure_weights.npz"))
+    cfg = C.DEFAULT
+    n = cfg["fine"] // cfg["ratio"]
+    announce(test="committed_model", n=n)
+    p = C.case_params(n, cfg["L"], C.viscosity(cfg))
+    u0 = C.initial_state(n, cfg["L"], 100, cfg["n_modes"], cfg["amplitude"], cfg["kmax"])
+    with od.MLP(params=w["params"]) as m, od.Solver(p) as s:
+        m.set_scaling(w["shift"], w["scale"], w["lo"], w["hi"])
+        m.set_output_scale(float(w["out_scale"]))
+        s.set_state(u0)
+        dt = s.cfl_dt(0.3)
+        s.attach_closure(m, update_interval=5, budget_kappa=1.0)
+        s.step(dt, 50)
+        mu = s.get_eddy_viscosity()
+        st = s.closure_stats()
+        assert np.isfinite(s.get_state()).all()
+    assert st["updates"] == 10 and np.all(mu >= 0.0) and (mu > 0).mean() > 0.5
+    # the eddy viscosity is a fraction of the molecular one at this Reynolds number
+    assert 0.0 < np.median(mu[mu > 0]) < C.viscosity(cfg)
