@@ -390,8 +390,16 @@ __device__ __forceinline__ FaceTask face_task(int lane) {
 struct Tile {
     int32_t e0, e1;
     bool v0, v1;
+    int32_t walls;  // bit 2 le + side: face yL/yR of element le is a slip wall
     Elem el0, el1;
 };
+
+// Whether the side `side` face of direction D of element le is a wall (only
+// y faces can be); needs no neighbour index.
+template <int D>
+__device__ __forceinline__ bool tile_wall(const Tile& t, int le, int side) {
+    return D == 1 && ((t.walls >> (2 * le + side)) & 1);
+}
 
 // Tile (t0, t1, t2) of a box: two elements adjacent along `orient`
 // (0 x, 1 y, 2 z); the tile coordinate along `orient` counts pairs.
@@ -422,6 +430,7 @@ __device__ __forceinline__ Tile make_tile_at(const StageArgs<UT, CT>& a, int ori
     const int d = v1 ? 1 : 0;
     t.el1 = make_elem(ex + (orient == 0 ? d : 0), ey + (orient == 1 ? d : 0), ez + (orient == 2 ? d : 0), a.nx, a.ny,
                       a.nz, a.wall);
+    t.walls = (t.el0.yl < 0 ? 1 : 0) | (t.el0.yr < 0 ? 2 : 0) | (t.el1.yl < 0 ? 4 : 0) | (t.el1.yr < 0 ? 8 : 0);
     return t;
 }
 
@@ -524,7 +533,10 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
     int base, stride;
     face_line(D, t.fp, base, stride);
     const Coef4<CT> c = side_coef(dev_ops<CT>(), t.side);
-    const int64_t nb = tile_nb<D>(tl, t.le, t.side);
+    // with the prefetch the neighbour index is not needed here, only whether
+    // the face is a wall (keeps 12 indices out of the registers)
+    const int64_t nb = PF ? 0 : tile_nb<D>(tl, t.le, t.side);
+    const bool wall = PF ? tile_wall<D>(tl, t.le, t.side) : nb < 0;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         CT own = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, br1_order(D));
@@ -533,8 +545,8 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
         if constexpr (PF)
             far = pf[kPfA + (t.le * 6 + t.f) * 80 + v * 16 + t.fp];
         else
-            far = nb >= 0 ? a.tr_in[nb * kTraceStride + (t.f ^ 1) * 80 + v * 16 + t.fp] : (CT)0;
-        CT nbv = nb >= 0 ? far : own * sign;
+            far = !wall ? a.tr_in[nb * kTraceStride + (t.f ^ 1) * 80 + v * 16 + t.fp] : (CT)0;
+        CT nbv = !wall ? far : own * sign;
         sJmp[t.le][t.f][v][t.fp] = ((CT)0.5) * (nbv + own) - own;
     }
 }
@@ -635,8 +647,8 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     CT own[5], oth[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) own[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
-    const int64_t nb = tile_nb<D>(tl, t.le, t.side);
-    const bool wall = nb < 0;
+    const int64_t nb = PF ? 0 : tile_nb<D>(tl, t.le, t.side);
+    const bool wall = PF ? tile_wall<D>(tl, t.le, t.side) : nb < 0;
     // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
     const int rf = D == 0 ? (right ? 6 : 7) : (D == 1 ? (right ? 2 : 3) : (right ? 4 : 5));
 #pragma unroll
@@ -727,12 +739,22 @@ __device__ __forceinline__ void write_traces2(const Ops<CT>& o, const CT (*sU)[5
 // ---------------------------------------------------------------------------
 // LDS of one tile (two elements).  The prefetch buffer comes first so that
 // its 16-byte LDS-DMA rows start on a 16-byte boundary.
+// With the prefetch (f32 compute) the gradient tile sG reuses the prefetched
+// traces and the BR1 jumps (960 floats each, contiguous): element 0's
+// gradients take the prefetch buffer, element 1's the jumps, written only
+// after both waves have finished reading them (a barrier).
 template <class CT>
 struct ViscLds {
-    float pf[std::is_same<CT, float>::value ? kPfB : 4];
-    CT sU[kEpb][5][64];
-    CT sG[kEpb][15][64];
+    static constexpr bool kPF = std::is_same<CT, float>::value;
+    float pf[kPF ? kPfB : 4];
     CT sJmp[kEpb][6][5][16];
+    CT sU[kEpb][5][64];
+    CT sG_own[kPF ? 0 : kEpb][15][64];  // zero-length when aliased
+    __device__ CT (*sG())[15][64] {
+        static_assert(!kPF || (kPfB == 15 * 64 && sizeof(sJmp) == kPfB * sizeof(float)), "sG aliasing layout");
+        if constexpr (kPF) return reinterpret_cast<CT(*)[15][64]>(pf);
+        else return sG_own;
+    }
 };
 
 // With the prefetch (f32 compute) the face-correction array sJ shares the
@@ -788,18 +810,19 @@ __device__ __forceinline__ void visc_body(const StageArgs<UT, CT>& a, const Tile
     __syncthreads();
     run_br1_jumps<PF>(a, L.sU, L.sJmp, tl, reinterpret_cast<const CT*>(L.pf), wave, pt);
     __syncthreads();
+    CT g[15];
+    if (tile_valid(tl, wave)) lane_gradient(a, L.sU[wave], L.sJmp[wave], i, j, k, g);
+    if constexpr (PF) __syncthreads();  // sG overwrites the jumps the other wave may still read
     if (tile_valid(tl, wave)) {
-        CT g[15];
-        lane_gradient(a, L.sU[wave], L.sJmp[wave], i, j, k, g);
 #pragma unroll
-        for (int q = 0; q < 15; ++q) L.sG[wave][q][pt] = g[q];
+        for (int q = 0; q < 15; ++q) L.sG()[wave][q][pt] = g[q];
     }
     __syncthreads();
     if (wave == 0) {
-        fv_pass<0>(a, L.sU, L.sG, tl, pt);
-        fv_pass<2>(a, L.sU, L.sG, tl, pt);
+        fv_pass<0>(a, L.sU, L.sG(), tl, pt);
+        fv_pass<2>(a, L.sU, L.sG(), tl, pt);
     } else {
-        fv_pass<1>(a, L.sU, L.sG, tl, pt);
+        fv_pass<1>(a, L.sU, L.sG(), tl, pt);
     }
 }
 
@@ -1049,6 +1072,7 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
     tl.e1 = uni(tl.e1);
     tl.el0 = uni(tl.el0);
     tl.el1 = uni(tl.el1);
+    tl.walls = uni(tl.walls);
     tl.v0 = ok;
     tl.v1 = ok && tl.v1;
 }

@@ -261,6 +261,8 @@ def test_committed_closure_model_runs_on_the_coarse_mesh():
         mu = s.get_eddy_viscosity()
         st = s.closure_stats()
         assert np.isfinite(s.get_state()).all()
-    assert st["updates"] == 10 and np.all(mu >= 0.0) and (mu > 0).mean() > 0.5
+    # elements outside the training range (snapshots t >= 5; this is t < 0.3) get mu_t = 0
+    assert st["updates"] == 10 and np.all(mu >= 0.0) and (mu > 0).mean() > 0.1
+    assert st["out_of_range"] > 0
     # the eddy viscosity is a fraction of the molecular one at this Reynolds number
     assert 0.0 < np.median(mu[mu > 0]) < C.viscosity(cfg)
