@@ -18,8 +18,9 @@ measures the ensemble instead: 24 independent periodic cubes [0, 2pi)^3 with
 16..40 elements per side (seed 7), P=3 NS f32 fast math, SLAU2, fixed mu
 1.6e-4, no body force, Mach 0.1 mean flow plus a divergence-free Fourier
 field per box (12..32 modes, amplitude 0.02..0.1, own seed), per-box dt from
-CFL 0.3; all boxes advance with one launch per kernel and stage, schedule
-picked by the autotuner.  Also one JSON line.
+CFL 0.3; the autotuner picks between one launch per kernel and stage for
+all boxes (several tile/grid schedules) and box-by-box launches inside the
+step graph; both are reported.  Also one JSON line.
 """
 import argparse
 import json
@@ -144,6 +145,8 @@ def run_ensemble(args):
         best, best_ms, table = B.autotune(dt, steps_per_trial=3)
         default_ms = [t["ms_per_step"] for t in table
                       if t["orient"] == 0 and t["pairs"] == 1 and t["order"] == 0 and t["groups_per_cu"] == 0][0]
+        batched = min((t for t in table if t["order"] != 2), key=lambda t: t["ms_per_step"])
+        box_by_box = [t["ms_per_step"] for t in table if t["order"] == 2][0]
         B.step(dt, args.warmup)
         B.synchronize()
         t0 = time.perf_counter()
@@ -169,7 +172,10 @@ def run_ensemble(args):
         },
         "schedule": best, "autotune_trials": len(table), "autotune_best_ms": round(best_ms, 5),
         "default_schedule_ms": round(default_ms, 5),
-        "per_box_launches_ms_per_step": round(ms_per_box, 5),
+        "one_launch_per_stage_best": {k: batched[k] for k in ("orient", "pairs", "order", "groups_per_cu")},
+        "one_launch_per_stage_best_ms": round(batched["ms_per_step"], 5),
+        "box_by_box_in_graph_ms": round(box_by_box, 5),
+        "per_box_launches_no_graph_ms_per_step": round(ms_per_box, 5),
         "wall_s_timed": round(wall, 4), "state_finite": finite,
         "mass_per_box_min_max": [float(sums[:, 0].min()), float(sums[:, 0].max())],
         "autotune_table": table,

@@ -125,9 +125,10 @@ const char* od_last_error(void);
  * wall-bounded boxes of different sizes sharing one od_params (physics,
  * scheme, precision; its nx..lz are ignored).  Every RK3 stage advances all
  * boxes with one launch of the gradient kernel and one of the update kernel
- * (a persistent grid walking the tiles of every box), captured as one
- * hipGraph per step.  Each box's result is byte-identical to the same box
- * run alone through od_step with the same dt, for every schedule. */
+ * (a grid walking the tiles of every box), or, with schedule order 2, with
+ * per-box launches; either way one hipGraph per step.  Each box's result is
+ * byte-identical to the same box run alone through od_step with the same
+ * dt, for every schedule. */
 typedef struct od_batch od_batch;
 
 typedef struct od_box {
@@ -140,7 +141,9 @@ typedef struct od_box {
 typedef struct od_schedule {
     int32_t orient;            /* tile = element pair along 0 x, 1 y, 2 z */
     int32_t pairs;             /* tiles per work-group: 1 (128 threads) or 2 (256) */
-    int32_t order;             /* 0 round robin; 1 contiguous tile range per XCD */
+    int32_t order;             /* 0 round robin; 1 contiguous tile range per XCD;
+                                  2 box by box: the single-box kernels launched per box
+                                  and stage inside the step graph (other fields ignored) */
     int32_t groups_per_cu;     /* 0: one work-group per work unit; k > 0: a
                                   persistent grid of k (at most the resident
                                   maximum) work-groups per CU */

@@ -128,6 +128,10 @@ od_status upload_table_t(od_batch* b, const double* dt, int orient) {
     HIP_TRY(hipMemcpy(b->table, t.data(), t.size() * sizeof(t[0]), hipMemcpyHostToDevice));
     b->table_dt.assign(dt, dt + b->boxes.size());
     b->table_orient = orient;
+    if (b->sched.order == 2 && b->graph) {  // per-box launches carry dt in their arguments
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
     return OD_OK;
 }
 
@@ -211,6 +215,15 @@ int max_occupancy(od_batch* b, int pairs) {
 }
 
 od_status enqueue_step(od_batch* b) {
+    if (b->sched.order == 2) {  // box by box with the single-box kernels
+        for (int s = 0; s < 3; ++s)
+            for (size_t i = 0; i < b->boxes.size(); ++i) {
+                const double dt = b->table_dt[i];
+                if (b->p.viscous && box_launch(b, (int)i, 0, s, dt)) return fail(OD_ERR_HIP, "od_visc_kernel launch failed");
+                if (box_launch(b, (int)i, 1, s, dt)) return fail(OD_ERR_HIP, "od_update_kernel launch failed");
+            }
+        return OD_OK;
+    }
     for (int s = 0; s < 3; ++s) {
         if (b->p.viscous && batch_launch(b, 0, s)) return fail(OD_ERR_HIP, "od_batch_visc_kernel launch failed");
         if (batch_launch(b, 1, s)) return fail(OD_ERR_HIP, "od_batch_update_kernel launch failed");
@@ -257,7 +270,7 @@ od_status check_schedule(const od_schedule* s) {
     if (!s) return fail(OD_ERR_INVALID, "schedule: null argument");
     if (s->orient < 0 || s->orient > 2) return fail(OD_ERR_INVALID, "schedule.orient must be 0, 1 or 2");
     if (s->pairs != 1 && s->pairs != 2) return fail(OD_ERR_INVALID, "schedule.pairs must be 1 or 2");
-    if (s->order != 0 && s->order != 1) return fail(OD_ERR_INVALID, "schedule.order must be 0 or 1");
+    if (s->order < 0 || s->order > 2) return fail(OD_ERR_INVALID, "schedule.order must be 0, 1 or 2");
     if (s->groups_per_cu < 0 || s->groups_per_cu > 64) return fail(OD_ERR_INVALID, "schedule.groups_per_cu must be 0..64");
     return OD_OK;
 }
@@ -520,6 +533,7 @@ od_status od_batch_autotune(od_batch* b, const double* dt, uint32_t steps_per_tr
     if (steps_per_trial < 1) return fail(OD_ERR_INVALID, "od_batch_autotune: steps_per_trial must be >= 1");
     HIP_TRY(hipSetDevice(b->device));
     std::vector<od_schedule> cand;
+    cand.push_back(od_schedule{0, 1, 2, 0});  // box by box
     for (int pairs = 1; pairs <= 2; ++pairs) {
         const int occ = max_occupancy(b, pairs);
         std::vector<int> gpcs = {0, 1};

@@ -973,8 +973,15 @@ __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
     visc_body(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
+// f32 compute: ask for 4 waves per SIMD (128 VGPRs; a few spill, measured
+// faster than 3 waves without); f64 compute needs ~270 VGPRs and keeps its
+// natural occupancy.
+template <class CT>
+constexpr int kMinWaves = sizeof(CT) == 4 ? 4 : 1;
+
 template <class UT, class CT, bool VISC>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void od_update_kernel(StageArgs<UT, CT> a) {
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWaves<CT>))) void od_update_kernel(
+    StageArgs<UT, CT> a) {
     __shared__ UpdateLds<CT, VISC> L;
     update_body<UT, CT, VISC>(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
@@ -1106,7 +1113,8 @@ __global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT
 }
 
 template <class UT, class CT, bool VISC, int PAIRS>
-__global__ __launch_bounds__(128 * PAIRS) void od_batch_update_kernel(StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
+__global__ __launch_bounds__(128 * PAIRS) __attribute__((amdgpu_waves_per_eu(kMinWaves<CT>))) void od_batch_update_kernel(
+    StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ UpdateLds<CT, VISC> L[PAIRS];
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;

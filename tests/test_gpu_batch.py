@@ -21,6 +21,7 @@ SCHEDULES = [
     dict(orient=1, pairs=2, order=1, groups_per_cu=1),
     dict(orient=2, pairs=1, order=1, groups_per_cu=2),
     dict(orient=0, pairs=2, order=0, groups_per_cu=1),
+    dict(orient=0, pairs=1, order=2, groups_per_cu=0),  # box by box inside the graph
 ]
 
 VARIANTS = {
@@ -30,6 +31,8 @@ VARIANTS = {
     "f64_ns_wall": dict(u_real=od.OD_F64, flux="slau2", y_wall=True, mu_ref=0.002),
     "mixed_euler": dict(u_real=od.OD_F64, compute_real=od.OD_F32, q_real=od.OD_F32, rhs_real=od.OD_F32,
                         flux="rusanov", viscous=False, body_force_x=0.0, rk_accum=1),
+    "mixed_ns_fast": dict(u_real=od.OD_F64, compute_real=od.OD_F32, q_real=od.OD_F32, rhs_real=od.OD_F32,
+                          flux="slau2", y_wall=True, mu_ref=0.002, rk_accum=1, fast_math=1),
 }
 
 
@@ -142,5 +145,29 @@ def test_per_box_launches_equal_batched():
         announce(test="per_box_timing")
         ms = b.time_steps(DT, 2, per_box=True)
         assert ms > 0
+        for q in range(len(BOXES)):
+            assert b.get_state(q).tobytes() == ref[q].tobytes()
+
+
+@pytest.mark.parametrize("order", [0, 2])
+def test_changing_dt_between_steps(order):
+    """A new dt rewrites the box table (batched) or re-captures the per-box
+    graph; both must follow the dt sequence of the boxes run alone."""
+    kw = VARIANTS["f32_fast_ns_wall"]
+    dt2 = [d * 0.5 for d in DT]
+    ref = []
+    for q, (nx, ny, nz, lx, ly, lz) in enumerate(BOXES):
+        with od.Solver(od.params(nx, ny, nz, lx, ly, lz, **kw)) as s:
+            s.set_state(initial(q, s.dtype))
+            s.step(DT[q], 2)
+            s.step(dt2[q], 1)
+            ref.append(s.get_state())
+    announce(test="dt_change", order=order)
+    with od.Batch(batch_params(kw), BOXES) as b:
+        b.set_schedule(orient=0, pairs=1, order=order, groups_per_cu=0)
+        for q in range(len(BOXES)):
+            b.set_state(q, initial(q, b.dtype))
+        b.step(DT, 2)
+        b.step(dt2, 1)
         for q in range(len(BOXES)):
             assert b.get_state(q).tobytes() == ref[q].tobytes()
