@@ -21,6 +21,10 @@ field per box (12..32 modes, amplitude 0.02..0.1, own seed), per-box dt from
 CFL 0.3; the autotuner picks between one launch per kernel and stage for
 all boxes (several tile/grid schedules) and box-by-box launches inside the
 step graph; both are reported.  Also one JSON line.
+
+    python bench.py --workload precision
+
+runs the precision study on the 8^3 periodic Euler case (one JSON line).
 """
 import argparse
 import json
@@ -183,6 +187,61 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def run_precision(args):
+    """Precision study on the 8^3 periodic Euler case (Rusanov, CFL 0.3, Fourier
+    IC 8 modes amp 1e-2 seed 0): per-variable error against the FP64 CPU
+    oracle after 10 and 100 steps for the all-f64 GPU pipeline, f64 state with
+    f32 flux/Riemann/derivatives and the RK accumulation in f64, and all-f32;
+    mass/energy drift and the mean rho*u profile after 2000 steps."""
+    import opendynamo_amd as od
+    from opendynamo_amd import ic
+    from oracle import oracle as O
+
+    n, L = 8, 2 * np.pi
+    base = dict(flux="rusanov", y_wall=False, viscous=False, body_force_x=0.0)
+    u0 = ic.fourier_state(n, n, n, L, L, L, u0=(0.1, 0.0, 0.0), amplitude=1e-2, n_modes=8, seed=0)
+    pipelines = {
+        "f64": dict(u_real=od.OD_F64),
+        "f64_state_f32_flux_f64_accum": dict(u_real=od.OD_F64, compute_real=od.OD_F32, q_real=od.OD_F32,
+                                             rhs_real=od.OD_F32, rk_accum=1),
+        "f32": dict(u_real=od.OD_F32),
+    }
+    po = O.params(n, n, n, L, L, L, n_threads=cpu_threads(), **base)
+    dt = O.cfl_dt(po, u0, CFL)
+    refs = {10: O.step(po, u0, dt, 10)}
+    refs[100] = O.step(po, refs[10], dt, 90)
+    names = ["rho", "rhou", "rhov", "rhow", "E"]
+    out = {"metric": "per-variable error vs the FP64 oracle, 8^3 periodic Euler (precision study)", "dt": dt,
+           "pipelines": {}}
+    long_states = {}
+    for name, kw in pipelines.items():
+        p = od.params(n, n, n, L, L, L, **base, **kw)
+        row = {}
+        with od.Solver(p) as s:
+            s.set_state(u0)
+            s0 = s.sums()
+            done = 0
+            for k in (10, 100):
+                s.step(dt, k - done)
+                done = k
+                g = s.get_state(np.float64).reshape(5, -1)
+                r = refs[k].reshape(5, -1)
+                row[f"after_{k}"] = {
+                    "max_rel": {nm: float(np.max(np.abs(g[v] - r[v])) / np.max(np.abs(r[v]))) for v, nm in enumerate(names)},
+                    "l2_rel": {nm: float(np.linalg.norm(g[v] - r[v]) / np.linalg.norm(r[v])) for v, nm in enumerate(names)},
+                }
+            s.step(dt, 2000 - done)
+            s2 = s.sums()
+            row["drift_2000"] = {"mass_rel": float((s2[0] - s0[0]) / s0[0]), "energy_rel": float((s2[4] - s0[4]) / s0[4])}
+            long_states[name] = s.get_state(np.float64).reshape(5, n, n, n, 4, 4, 4)
+        out["pipelines"][name] = row
+    # mean rho*u profile over y (x, z and in-element x, z points averaged), against the all-f64 GPU run
+    prof = {k: v[1].mean(axis=(0, 2, 3, 5)).ravel() for k, v in long_states.items()}  # [ey][j]
+    for k in prof:
+        out["pipelines"][k]["mean_rhou_profile_max_abs_diff_vs_f64_2000"] = float(np.max(np.abs(prof[k] - prof["f64"])))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,7 +251,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy of known size, the reference "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
-    ap.add_argument("--workload", choices=["channel48", "ensemble"], default="channel48")
+    ap.add_argument("--workload", choices=["channel48", "ensemble", "precision"], default="channel48")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -200,6 +259,9 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         run_ensemble(args)
+        return
+    if args.workload == "precision":
+        run_precision(args)
         return
 
     import opendynamo_amd as od
