@@ -391,7 +391,7 @@ struct WaveLds {
 template <class CT>
 __global__ __launch_bounds__(256) void od_mlp_infer_kernel(DevNet net, Scaling sc, const float* __restrict__ feat,
                                                            int64_t E, CT* __restrict__ mu_t,
-                                                           float* __restrict__ eps, int* __restrict__ out_of_range) {
+                                                           float* __restrict__ eps, float* __restrict__ out_of_range) {
     __shared__ NetLds L;
     __shared__ WaveLds W[4];
     load_net(net, L, nullptr);
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void od_mlp_infer_kernel(DevNet net, Scaling s
                 const float m = in ? fe[12] * fe[13] * (sc.out_scale * softplus(z4[q])) : 0.0f;
                 mu_t[e] = (CT)m;
                 eps[e] = m * fe[15];
-                out_of_range[e] = in ? 0 : 1;
+                out_of_range[e] = in ? 0.0f : 1.0f;
             }
         }
     }
@@ -648,28 +648,33 @@ __global__ void od_mlp_adam_kernel(float* __restrict__ p, const float* __restric
     p[q] -= lr * mh / (sqrtf(vh) + 1e-8f);
 }
 
-// Deterministic sum of two float arrays of length n (one block of 256).
-__global__ __launch_bounds__(256) void od_sum2_kernel(const float* __restrict__ a, int as, const float* __restrict__ b,
-                                                      int bs, int64_t n, double* __restrict__ out) {
-    __shared__ double ra[256], rb[256];
-    double sa = 0.0, sb = 0.0;
+// Deterministic sums of three float arrays of length n (one block of 256,
+// fixed order): out[0..2].
+__global__ __launch_bounds__(256) void od_sum3_kernel(const float* __restrict__ a, int as, const float* __restrict__ b,
+                                                      const float* __restrict__ c, int64_t n, double* __restrict__ out) {
+    __shared__ double ra[256], rb[256], rc[256];
+    double sa = 0.0, sb = 0.0, sc = 0.0;
     for (int64_t q = threadIdx.x; q < n; q += 256) {
         sa += (double)a[q * as];
-        sb += (double)b[q * bs];
+        sb += (double)b[q];
+        sc += (double)c[q];
     }
     ra[threadIdx.x] = sa;
     rb[threadIdx.x] = sb;
+    rc[threadIdx.x] = sc;
     __syncthreads();
     for (int h = 128; h >= 1; h >>= 1) {
         if ((int)threadIdx.x < h) {
             ra[threadIdx.x] += ra[threadIdx.x + h];
             rb[threadIdx.x] += rb[threadIdx.x + h];
+            rc[threadIdx.x] += rc[threadIdx.x + h];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         out[0] = ra[0];
         out[1] = rb[0];
+        out[2] = rc[0];
     }
 }
 
@@ -784,10 +789,9 @@ FeatConsts feat_consts(const od_ctx* c) {
 
 od_status ensure_closure_buffers(od_ctx* c) {
     if (!c->feat) {
-        if (hipMalloc(&c->feat, (size_t)c->E * FS * sizeof(float) + (size_t)c->E * (sizeof(float) + sizeof(int))) !=
-            hipSuccess)
+        if (hipMalloc(&c->feat, (size_t)c->E * (FS + 2) * sizeof(float)) != hipSuccess)
             return fail(OD_ERR_NOMEM, "closure buffers: allocation failed");
-        c->device_bytes += (size_t)c->E * (FS * sizeof(float) + sizeof(float) + sizeof(int));
+        c->device_bytes += (size_t)c->E * (FS + 2) * sizeof(float);
     }
     if (!c->red) {
         HIP_TRY(hipMalloc((void**)&c->red, 4 * sizeof(double)));
@@ -832,7 +836,7 @@ od_status od_closure_update_internal(od_ctx* c, double dt_interval) {
     if (st != OD_OK) return st;
     if (launch_features(c)) return fail(OD_ERR_HIP, "od_features_kernel launch failed");
     float* eps = (float*)c->feat + (size_t)c->E * FS;
-    int* oor = (int*)(eps + c->E);
+    float* oor = eps + c->E;
     const unsigned blocks = (unsigned)std::min<int64_t>(1024, (c->E + 63) / 64);
     const DevNet net = dev_net(c->mlp);
     if (c->cbytes == 4)
@@ -842,18 +846,14 @@ od_status od_closure_update_internal(od_ctx* c, double dt_interval) {
         hipLaunchKernelGGL(od_mlp_infer_kernel<double>, dim3(blocks), dim3(256), 0, c->stream, net, c->mlp->sc,
                            (const float*)c->feat, c->E, (double*)c->mu_t, eps, oor);
     if (check()) return fail(OD_ERR_HIP, "od_mlp_infer_kernel launch failed");
-    // kinetic energy and model dissipation rate; out-of-range count
-    hipLaunchKernelGGL(od_sum2_kernel, dim3(1), dim3(256), 0, c->stream, (const float*)c->feat + 14, FS,
-                       (const float*)eps, 1, c->E, c->red);
-    if (check()) return fail(OD_ERR_HIP, "od_sum2_kernel launch failed");
-    std::vector<int> oor_h;  // counted on the host: small (E ints) and only at updates
-    oor_h.resize(c->E);
-    HIP_TRY(hipMemcpyAsync(c->red_host, c->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(oor_h.data(), oor, c->E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // kinetic energy, model dissipation rate and out-of-range count (exact in f64)
+    hipLaunchKernelGGL(od_sum3_kernel, dim3(1), dim3(256), 0, c->stream, (const float*)c->feat + 14, FS,
+                       (const float*)eps, (const float*)oor, c->E, c->red);
+    if (check()) return fail(OD_ERR_HIP, "od_sum3_kernel launch failed");
+    HIP_TRY(hipMemcpyAsync(c->red_host, c->red, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const double ke = c->red_host[0], eps_now = c->red_host[1];
-    int64_t n_oor = 0;
-    for (int v : oor_h) n_oor += v;
+    const int64_t n_oor = (int64_t)c->red_host[2];
     // budget check over the interval that just ended
     bool active = true;
     if (c->closure.budget_kappa > 0.0 && c->ke_prev >= 0.0 && c->closure_active) {

@@ -37,7 +37,7 @@ struct od_ctx {
     double eps_prev = 0.0;        // model dissipation rate applied since then
     bool closure_active = false;  // false: budget fallback, plain FR until the next update
     double closure_stats[6] = {0, 0, 0, 0, 0, 0};
-    void* feat = nullptr;         // [E][16] features, then eps [E] f32, out-of-range [E] int
+    void* feat = nullptr;         // [E][16] features, then eps [E] and out-of-range flags [E] (f32)
     double* red = nullptr;        // budget-check reductions (device)
     double* red_host = nullptr;   // pinned copy
 };
