@@ -115,26 +115,20 @@ __device__ __forceinline__ void euler_flux(int d, const C* s, C* f) {  // :7-21
 }
 
 // :23-102.  g[3*var + dir]; G[a][b] = d(u_a)/d(x_b).
+// The direction-independent part of the three directional viscous fluxes of
+// a point: velocity, stress tensor and the heat flux along each direction.
+// Every entry is the expression the directional flux evaluates, so the
+// fluxes assembled from it are bitwise those of compute_viscous_flux_{x,y,z};
+// computing it once frees the 15 gradient values before the three
+// directions are assembled (fewer live registers in the volume pass).
 template <class C>
-__device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu, C* f) {
-    C rho = s[0];
-    C u[3] = {dv(s[1], rho), dv(s[2], rho), dv(s[3], rho)};
-    C G[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) G[a][b] = dv(g[3 * (a + 1) + b] - u[a] * g[b], rho);
-    C div = G[0][0] + G[1][1] + G[2][2];
-    const C two = (C)2.0, tt = ((C)2.0) / ((C)3.0);
-    C t00 = mu * (two * G[0][0] - tt * div);
-    C t11 = mu * (two * G[1][1] - tt * div);
-    C t22 = mu * (two * G[2][2] - tt * div);
-    C t01 = mu * (G[0][1] + G[1][0]);
-    C t02 = mu * (G[0][2] + G[2][0]);
-    C t12 = mu * (G[1][2] + G[2][1]);
-    C r0 = sel3(t00, t01, t02, d), r1 = sel3(t01, t11, t12, d), r2 = sel3(t02, t12, t22, d);
-    C ke = dv(((C)0.5) * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]), rho);
-    C p = gamma_m1<C>() * (s[4] - ke);
+struct ViscPoint {
+    C u[3];
+    C t00, t11, t22, t01, t02, t12;
+    C q[3];
+};
+template <class C>
+__device__ __forceinline__ C heat_flux(int d, const C* s, const C* g, C mu, C rho, C ke, C p) {
     C gd = sel3(g[0], g[1], g[2], d);
     C gu = sel3(g[3], g[4], g[5], d);
     C gv = sel3(g[6], g[7], g[8], d);
@@ -143,12 +137,47 @@ __device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu
     C dke = dv(s[1] * gu + s[2] * gv + s[3] * gw, rho) - dv(ke * gd, rho);
     C dp = gamma_m1<C>() * (ge - dke);
     C coeff = dv(mu * ((C)1.4), ((C)0.72) * gamma_m1<C>());
-    C q = -coeff * (dv(dp, rho) - dv(p * gd, rho * rho));
+    return -coeff * (dv(dp, rho) - dv(p * gd, rho * rho));
+}
+// dirs: bit d set = the heat flux along d is needed.
+template <class C>
+__device__ __forceinline__ ViscPoint<C> viscous_point(const C* s, const C* g, C mu, int dirs) {
+    ViscPoint<C> vp;
+    C rho = s[0];
+    vp.u[0] = dv(s[1], rho);
+    vp.u[1] = dv(s[2], rho);
+    vp.u[2] = dv(s[3], rho);
+    C G[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) G[a][b] = dv(g[3 * (a + 1) + b] - vp.u[a] * g[b], rho);
+    C div = G[0][0] + G[1][1] + G[2][2];
+    const C two = (C)2.0, tt = ((C)2.0) / ((C)3.0);
+    vp.t00 = mu * (two * G[0][0] - tt * div);
+    vp.t11 = mu * (two * G[1][1] - tt * div);
+    vp.t22 = mu * (two * G[2][2] - tt * div);
+    vp.t01 = mu * (G[0][1] + G[1][0]);
+    vp.t02 = mu * (G[0][2] + G[2][0]);
+    vp.t12 = mu * (G[1][2] + G[2][1]);
+    C ke = dv(((C)0.5) * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]), rho);
+    C p = gamma_m1<C>() * (s[4] - ke);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vp.q[d] = (dirs >> d) & 1 ? heat_flux(d, s, g, mu, rho, ke, p) : (C)0;
+    return vp;
+}
+template <class C>
+__device__ __forceinline__ void viscous_flux_dir(int d, const ViscPoint<C>& vp, C* f) {
+    C r0 = sel3(vp.t00, vp.t01, vp.t02, d), r1 = sel3(vp.t01, vp.t11, vp.t12, d), r2 = sel3(vp.t02, vp.t12, vp.t22, d);
     f[0] = (C)0.0;
     f[1] = r0;
     f[2] = r1;
     f[3] = r2;
-    f[4] = u[0] * r0 + u[1] * r1 + u[2] * r2 - q;
+    f[4] = vp.u[0] * r0 + vp.u[1] * r1 + vp.u[2] * r2 - vp.q[d];
+}
+template <class C>
+__device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu, C* f) {
+    viscous_flux_dir(d, viscous_point(s, g, mu, 1 << d), f);
 }
 
 template <class C>
@@ -590,6 +619,10 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
             gy = (CT)(float)gy;
             gz = (CT)(float)gz;
         }
+        // Finish variable v before the next variable's LDS reads: left to
+        // itself the compiler issues all 90 reads first and computes at the
+        // first use, which needs ~140 VGPRs (and spills at 4 waves/SIMD).
+        asm volatile("" : "+v"(gx), "+v"(gy), "+v"(gz)::"memory");
         g[3 * v + 0] = gx;
         g[3 * v + 1] = gy;
         g[3 * v + 2] = gz;
@@ -865,13 +898,15 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 
     // Volume flux divergence, x then y then z (rhs.cu:229-266, 490-510).
     CT rs[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
+    ViscPoint<CT> vp;
+    if (VISC) vp = viscous_point(st, g, mu, 7);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         CT fe[5], F[5];
         euler_flux(d, st, fe);
         if (VISC) {
             CT fvv[5];
-            viscous_flux(d, st, g, mu, fvv);
+            viscous_flux_dir(d, vp, fvv);
 #pragma unroll
             for (int v = 0; v < 5; ++v) F[v] = fe[v] - fvv[v];
         } else {
