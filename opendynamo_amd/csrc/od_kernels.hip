@@ -74,10 +74,6 @@ __device__ __forceinline__ C sq(C x) {
 }
 
 template <class T>
-__device__ __forceinline__ T sel4(T a0, T a1, T a2, T a3, int i) {
-    return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
-}
-template <class T>
 __device__ __forceinline__ T sel3(T a0, T a1, T a2, int d) {
     return d == 0 ? a0 : (d == 1 ? a1 : a2);
 }
@@ -395,10 +391,36 @@ template <class C>
 struct Coef4 {
     C c0, c1, c2, c3;
 };
+
+// The FR operators in LDS, laid out so that a lane reads the row or the side
+// it needs directly: selecting among 4 wave-uniform scalars costs 3
+// v_cndmask and 2 v_mov per coefficient and lane (one scalar operand per
+// VALU instruction).
 template <class C>
-__device__ __forceinline__ Coef4<C> side_coef(const Ops<C>& o, bool right) {
-    return {right ? o.eR[0] : o.eL[0], right ? o.eR[1] : o.eL[1], right ? o.eR[2] : o.eL[2],
-            right ? o.eR[3] : o.eL[3]};
+struct alignas(16) CoefTable {
+    C D[4][4];  // D[i][m] = l_m'(x_i)
+    C g[4][2];  // {g_left'(x_i), g_right'(x_i)}
+    C e[2][4];  // {l_m(-1)}, {l_m(+1)}
+};
+template <class C>
+__device__ __forceinline__ CoefTable<C>& coef_lds() {
+    __shared__ CoefTable<C> t;
+    return t;
+}
+// Threads 0..31 of the work-group copy the table from constant memory; the
+// kernel's first barrier orders the copy before every read.
+template <class C>
+__device__ __forceinline__ void coef_fill(int tid) {
+    static_assert(sizeof(Ops<C>) == 32 * sizeof(C) && sizeof(CoefTable<C>) == 32 * sizeof(C), "table layout");
+    if (tid >= 32) return;
+    // Ops order: D[16], eL[4], eR[4], gL[4], gR[4]
+    const int src = tid < 16 ? tid : (tid < 24 ? 24 + ((tid - 16) & 1) * 4 + ((tid - 16) >> 1) : tid - 8);
+    reinterpret_cast<C*>(&coef_lds<C>())[tid] = reinterpret_cast<const C*>(&dev_ops<C>())[src];
+}
+template <class C>
+__device__ __forceinline__ Coef4<C> side_coef(bool right) {
+    const C* r = coef_lds<C>().e[right];
+    return {r[0], r[1], r[2], r[3]};
 }
 
 // Task of a direction-D face pass: lane -> (local element, side, face point).
@@ -561,7 +583,7 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
     face_line(D, t.fp, base, stride);
-    const Coef4<CT> c = side_coef(dev_ops<CT>(), t.side);
+    const Coef4<CT> c = side_coef<CT>(t.side);
     // with the prefetch the neighbour index is not needed here, only whether
     // the face is a wall (keeps 12 indices out of the registers)
     const int64_t nb = PF ? 0 : tile_nb<D>(tl, t.le, t.side);
@@ -585,19 +607,15 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
 template <class UT, class CT>
 __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const CT (*sU)[64], const CT (*sJmp)[5][16],
                                               int i, int j, int k, CT* g) {
-    const Ops<CT>& o = dev_ops<CT>();
-    CT Di[4], Dj[4], Dk[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        Di[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], i);
-        Dj[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], j);
-        Dk[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], k);
-    }
+    const CoefTable<CT>& o = coef_lds<CT>();
+    const CT* Di = o.D[i];
+    const CT* Dj = o.D[j];
+    const CT* Dk = o.D[k];
     const int xb = 4 * j + 16 * k, yb = i + 16 * k, zb = i + 4 * j;
     const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
-    const CT gLi = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], i), gRi = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], i);
-    const CT gLj = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], j), gRj = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], j);
-    const CT gLk = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], k), gRk = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], k);
+    const CT gLi = o.g[i][0], gRi = o.g[i][1];
+    const CT gLj = o.g[j][0], gRj = o.g[j][1];
+    const CT gLk = o.g[k][0], gRk = o.g[k][1];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         CT dz = 0, dxi = 0, deta = 0;
@@ -652,7 +670,7 @@ __device__ __forceinline__ void fv_pass(const StageArgs<UT, CT>& a, const CT (*s
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
     face_line(D, t.fp, base, stride);
-    const Coef4<CT> c = side_coef(dev_ops<CT>(), t.side);
+    const Coef4<CT> c = side_coef<CT>(t.side);
     CT st[5], gt[15], fvv[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) st[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
@@ -675,7 +693,7 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     if (!tile_valid(tl, t.le)) return;
     int base, stride;
     face_line(D, t.fp, base, stride);
-    const Coef4<CT> c = side_coef(dev_ops<CT>(), t.side);
+    const Coef4<CT> c = side_coef<CT>(t.side);
     const bool right = t.side;
     CT own[5], oth[5];
 #pragma unroll
@@ -741,8 +759,8 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
 // (pairwise and sequential orders), wave 1 the y and z traces; each wave
 // covers both elements in two passes.
 template <class CT>
-__device__ __forceinline__ void write_traces2(const Ops<CT>& o, const CT (*sU)[5][64], CT* tr_out, const Tile& tl,
-                                              int wave, int lane, bool need_pairwise_x) {
+__device__ __forceinline__ void write_traces2(const CT (*sU)[5][64], CT* tr_out, const Tile& tl, int wave, int lane,
+                                              bool need_pairwise_x) {
     const int fsel = lane >> 4, fp = lane & 15;
     int f, d, order;
     if (wave == 0) {
@@ -757,7 +775,7 @@ __device__ __forceinline__ void write_traces2(const Ops<CT>& o, const CT (*sU)[5
     if (wave == 0 && !need_pairwise_x && fsel < 2) return;
     int base, stride;
     face_line(d, fp, base, stride);
-    const Coef4<CT> c = side_coef(o, f & 1);
+    const Coef4<CT> c = side_coef<CT>(f & 1);
 #pragma unroll
     for (int le = 0; le < kEpb; ++le) {
         if (!tile_valid(tl, le)) continue;
@@ -823,7 +841,7 @@ __device__ __forceinline__ void trace_body(const StageArgs<UT, CT>& a, const Til
         for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
     }
     __syncthreads();
-    write_traces2(dev_ops<CT>(), sU, a.tr_out, tl, wave, pt, true);
+    write_traces2(sU, a.tr_out, tl, wave, pt, true);
 }
 
 // Corrected gradients of the tile's elements and the viscous fluxes at
@@ -870,7 +888,6 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     const bool mine = tile_valid(tl, wave);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t gbase = tile_eidx(tl, wave) * 64 + pt;
-    const Ops<CT>& o = dev_ops<CT>();
     if constexpr (PF) prefetch_issue(a, tl, L.pf, VISC ? kPfChunksFull : kPfChunksEuler, wave, pt);
 
     constexpr bool SAME = std::is_same<UT, CT>::value;  // sU holds the state exactly
@@ -921,9 +938,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
         const int base = d == 0 ? 4 * j + 16 * k : (d == 1 ? i + 16 * k : i + 4 * j);
         const int stride = d == 0 ? 1 : (d == 1 ? 4 : 16);
         const CT jac = d == 0 ? a.jx : (d == 1 ? a.jy : a.jz);
-        CT Dd[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) Dd[m] = sel4(o.D[m], o.D[4 + m], o.D[8 + m], o.D[12 + m], idx);
+        const CT* Dd = coef_lds<CT>().D[idx];
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             CT dd = 0;
@@ -951,9 +966,10 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     __syncthreads();
 
     // FR correction and stage update (rk.cu:12-25, 63-82).
-    const CT gLi = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], i), gRi = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], i);
-    const CT gLj = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], j), gRj = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], j);
-    const CT gLk = sel4(o.gL[0], o.gL[1], o.gL[2], o.gL[3], k), gRk = sel4(o.gR[0], o.gR[1], o.gR[2], o.gR[3], k);
+    const CoefTable<CT>& o = coef_lds<CT>();
+    const CT gLi = o.g[i][0], gRi = o.g[i][1];
+    const CT gLj = o.g[j][0], gRj = o.g[j][1];
+    const CT gLk = o.g[k][0], gRk = o.g[k][1];
     const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
     UT uprv[5], ucr[5];
     CT r[5];
@@ -993,18 +1009,20 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)unew[v];
     __syncthreads();
-    write_traces2(o, L.sU, a.tr_out, tl, wave, pt, VISC);
+    write_traces2(L.sU, a.tr_out, tl, wave, pt, VISC);
 }
 
 template <class UT, class CT>
 __global__ __launch_bounds__(128) void od_trace_kernel(StageArgs<UT, CT> a) {
     __shared__ CT sU[kEpb][5][64];
+    coef_fill<CT>(threadIdx.x);
     trace_body(a, make_tile(a), sU, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
 template <class UT, class CT>
 __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
     __shared__ ViscLds<CT> L;
+    coef_fill<CT>(threadIdx.x);
     visc_body(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
@@ -1018,6 +1036,7 @@ template <class UT, class CT, bool VISC>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWaves<CT>))) void od_update_kernel(
     StageArgs<UT, CT> a) {
     __shared__ UpdateLds<CT, VISC> L;
+    coef_fill<CT>(threadIdx.x);
     update_body<UT, CT, VISC>(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
@@ -1122,6 +1141,7 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
 template <class UT, class CT, int PAIRS>
 __global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ ViscLds<CT> L[PAIRS];
+    coef_fill<CT>(threadIdx.x);
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
     if ((int)gridDim.x >= b.n_units && b.order == 0) {  // one unit per work-group: no loop
@@ -1151,6 +1171,7 @@ template <class UT, class CT, bool VISC, int PAIRS>
 __global__ __launch_bounds__(128 * PAIRS) __attribute__((amdgpu_waves_per_eu(kMinWaves<CT>))) void od_batch_update_kernel(
     StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ UpdateLds<CT, VISC> L[PAIRS];
+    coef_fill<CT>(threadIdx.x);
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
     if ((int)gridDim.x >= b.n_units && b.order == 0) {  // one unit per work-group: no loop
