@@ -140,7 +140,10 @@ int launch_kind(od_ctx* c, int kind, int stage, double dt, int mode) {
     if (kind == 2) {
         a.uin = (const UT*)c->u[0];
         a.tr_out = (CT*)c->tr[0];
-        return od::launch_trace<UT, CT>(a, c->E, c->stream);  // pure data movement: one build serves both
+        // the fast build's traces (one x order, fused arithmetic) come from its own kernel
+        if constexpr (sizeof(CT) == 4)
+            if (c->p.fast_math) return od::fastmath::launch_trace<UT, CT>(a, c->E, c->stream);
+        return od::launch_trace<UT, CT>(a, c->E, c->stream);
     }
     if constexpr (sizeof(CT) == 4) {
         if (c->p.fast_math) {

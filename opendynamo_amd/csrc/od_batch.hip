@@ -79,6 +79,9 @@ int box_launch_t(od_batch* b, int i, int kind, int stage, double dt) {
     if (kind == 2) {
         a.uin = a.uprev;  // u0 of the box
         a.tr_out = (CT*)a.tr_in;  // tr[0] of the box (stage 0 reads tr[0])
+        // the fast build's traces (one x order, fused arithmetic) come from its own kernel
+        if constexpr (sizeof(CT) == 4)
+            if (b->p.fast_math) return od::fastmath::launch_trace<UT, CT>(a, E, b->stream);
         return od::launch_trace<UT, CT>(a, E, b->stream);
     }
     if constexpr (sizeof(CT) == 4) {

@@ -368,6 +368,17 @@ __device__ __forceinline__ void face_line(int d, int fp, int& base, int& stride)
 // (rhs.cu:131-138, 540-543).
 template <class C>
 __device__ __forceinline__ C trace4(const C* q, int base, int stride, C c0, C c1, C c2, C c3, int order) {
+#if OD_FAST
+    // Fast build: explicit fused forms, so that a trace comes out bitwise the
+    // same in every kernel that forms it (the trace kernel and the update
+    // kernel write the traces the next stage reads) whatever the compiler
+    // would contract in each context.
+    if constexpr (sizeof(C) == 4) {
+        const C q0 = q[base], q1 = q[base + stride], q2 = q[base + 2 * stride], q3 = q[base + 3 * stride];
+        if (order == 0) return __builtin_fmaf(q1, c1, q0 * c0) + __builtin_fmaf(q3, c3, q2 * c2);
+        return __builtin_fmaf(q3, c3, __builtin_fmaf(q2, c2, __builtin_fmaf(q1, c1, q0 * c0)));
+    }
+#endif
     C t0 = q[base] * c0, t1 = q[base + stride] * c1, t2 = q[base + 2 * stride] * c2, t3 = q[base + 3 * stride] * c3;
     if (order == 0) return (t0 + t1) + (t2 + t3);
     if (order == 1) {
@@ -385,7 +396,13 @@ __device__ __forceinline__ C trace4(const C* q, int base, int stride, C c0, C c1
 __device__ __forceinline__ constexpr int br1_order(int d) { return d == 2 ? 1 : 0; }
 // Order of the Riemann inputs and gradient slices (rhs.cu:126-180, 540-543,
 // 663-674, 716-727): x sequential from the first product, y pairwise, z from 0.
-__device__ __forceinline__ constexpr int face_order(int d) { return d == 0 ? 2 : (d == 1 ? 0 : 1); }
+// The fast build keeps a single x trace per face (faces 0/1, pairwise order)
+// for BR1 and the Riemann flux alike; the strict build keeps the reference's
+// two orders (0/1 pairwise for BR1, 6/7 sequential for the interface flux).
+// Both sides of a face still see the same two traces, so the interface flux
+// stays single-valued (conservative) either way.
+constexpr bool kOneX = OD_FAST;
+__device__ __forceinline__ constexpr int face_order(int d) { return d == 0 ? (kOneX ? 0 : 2) : (d == 1 ? 0 : 1); }
 
 template <class C>
 struct Coef4 {
@@ -537,6 +554,7 @@ __device__ __forceinline__ bool prefetch_chunk(const StageArgs<UT, CT>& a, const
         const int64_t own = tile_eidx(tl, le), nb = tile_nb_rt(tl, le, f);
         src = a.tr_in + (nb >= 0 ? nb : own) * kTraceStride + (f ^ 1) * 80 + c * 4;
     } else if (id < 320) {  // B
+        if (kOneX) return false;  // the x Riemann inputs are group A's
         const int r = id - 240, le = r / 40, f = (r % 40) / 20, c = r % 20;
         if (!tile_valid(tl, le)) return false;
         const int64_t nb = tile_nb_rt(tl, le, f);  // x is periodic: always a neighbour
@@ -701,12 +719,12 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     const int64_t nb = PF ? 0 : tile_nb<D>(tl, t.le, t.side);
     const bool wall = PF ? tile_wall<D>(tl, t.le, t.side) : nb < 0;
     // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
-    const int rf = D == 0 ? (right ? 6 : 7) : (D == 1 ? (right ? 2 : 3) : (right ? 4 : 5));
+    const int rf = (D == 0 && !kOneX) ? (right ? 6 : 7) : (t.f ^ 1);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         CT far;
         if constexpr (PF)
-            far = D == 0 ? pf[kPfB + (t.le * 2 + t.side) * 80 + v * 16 + t.fp]
+            far = (D == 0 && !kOneX) ? pf[kPfB + (t.le * 2 + t.side) * 80 + v * 16 + t.fp]
                          : pf[kPfA + (t.le * 6 + t.f) * 80 + v * 16 + t.fp];
         else
             far = wall ? own[v] : a.tr_in[nb * kTraceStride + rf * 80 + v * 16 + t.fp];
@@ -756,12 +774,24 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
 }
 
 // Face traces of the state tile -> tr_out.  Wave 0 writes the x traces
-// (pairwise and sequential orders), wave 1 the y and z traces; each wave
-// covers both elements in two passes.
+// (strict build: pairwise and sequential orders, two passes; fast build:
+// the pairwise ones of both elements in one pass), wave 1 the y and z
+// traces of both elements in two passes.
 template <class CT>
 __device__ __forceinline__ void write_traces2(const CT (*sU)[5][64], CT* tr_out, const Tile& tl, int wave, int lane,
                                               bool need_pairwise_x) {
     const int fsel = lane >> 4, fp = lane & 15;
+    if (kOneX && wave == 0) {
+        const int le = fsel >> 1, f = fsel & 1;
+        if (!tile_valid(tl, le)) return;
+        int base, stride;
+        face_line(0, fp, base, stride);
+        const Coef4<CT> c = side_coef<CT>(f);
+        CT* dst = tr_out + tile_eidx(tl, le) * kTraceStride + f * 80 + fp;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) dst[v * 16] = trace4(sU[le][v], base, stride, c.c0, c.c1, c.c2, c.c3, 0);
+        return;
+    }
     int f, d, order;
     if (wave == 0) {
         f = fsel < 2 ? fsel : fsel + 4;  // 0, 1 pairwise; 6, 7 sequential from the first product
