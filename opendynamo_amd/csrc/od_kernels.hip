@@ -643,6 +643,12 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
         for (int m = 0; m < 4; ++m) dxi += Di[m] * sU[v][xb + m];
 #pragma unroll
         for (int m = 0; m < 4; ++m) deta += Dj[m] * sU[v][yb + 4 * m];
+#if OD_FAST
+        // one Jacobian factor per direction
+        CT gx = (dxi + sJmp[0][v][fpx] * gLi + sJmp[1][v][fpx] * gRi) * a.jx;
+        CT gy = (deta + sJmp[2][v][fpy] * gLj + sJmp[3][v][fpy] * gRj) * a.jy;
+        CT gz = (dz + sJmp[4][v][fpz] * gLk + sJmp[5][v][fpz] * gRk) * a.jz;
+#else
         CT gx = dxi * a.jx, gy = deta * a.jy, gz = dz * a.jz;
         gx += sJmp[0][v][fpx] * gLi * a.jx;
         gx += sJmp[1][v][fpx] * gRi * a.jx;
@@ -650,6 +656,7 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
         gy += sJmp[3][v][fpy] * gRj * a.jy;
         gz += sJmp[4][v][fpz] * gLk * a.jz;
         gz += sJmp[5][v][fpz] * gRk * a.jz;
+#endif
         if (a.round_q) {  // Q_REAL round trip (rhs.cu:457-461), narrows only when CT is double
             gx = (CT)(float)gx;
             gy = (CT)(float)gy;
@@ -997,9 +1004,16 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 
     // FR correction and stage update (rk.cu:12-25, 63-82).
     const CoefTable<CT>& o = coef_lds<CT>();
+#if OD_FAST
+    // correction slopes with the Jacobian folded in
+    const CT gLi = o.g[i][0] * a.jx, gRi = o.g[i][1] * a.jx;
+    const CT gLj = o.g[j][0] * a.jy, gRj = o.g[j][1] * a.jy;
+    const CT gLk = o.g[k][0] * a.jz, gRk = o.g[k][1] * a.jz;
+#else
     const CT gLi = o.g[i][0], gRi = o.g[i][1];
     const CT gLj = o.g[j][0], gRj = o.g[j][1];
     const CT gLk = o.g[k][0], gRk = o.g[k][1];
+#endif
     const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
     UT uprv[5], ucr[5];
     CT r[5];
@@ -1013,9 +1027,15 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const CT(*J)[5][16] = L.sJ()[wave];
+#if OD_FAST
+        CT cx = -(J[0][v][fpx] * gLi + J[1][v][fpx] * gRi);
+        CT cy = -(J[2][v][fpy] * gLj + J[3][v][fpy] * gRj);
+        CT cz = -(J[4][v][fpz] * gLk + J[5][v][fpz] * gRk);
+#else
         CT cx = -(J[0][v][fpx] * gLi * a.jx) - (J[1][v][fpx] * gRi * a.jx);
         CT cy = -(J[2][v][fpy] * gLj * a.jy) - (J[3][v][fpy] * gRj * a.jy);
         CT cz = -(J[4][v][fpz] * gLk * a.jz) - (J[5][v][fpz] * gRk * a.jz);
+#endif
         CT rt = r[v] + (cx + cy + cz);
         if (a.mode == 1) {
             if (mine) a.dbg[v * NP + gbase] = rt;
