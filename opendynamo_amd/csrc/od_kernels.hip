@@ -392,17 +392,22 @@ __device__ __forceinline__ C trace4(const C* q, int base, int stride, C c0, C c1
     return t0 + t1 + t2 + t3;
 }
 
+// The fast build forms every trace in the pairwise (fused) form and keeps a
+// single x trace per face (faces 0/1) for BR1 and the Riemann flux alike;
+// the strict build keeps the reference's orders below, with two x traces
+// (0/1 pairwise for BR1, 6/7 sequential for the interface flux).  Both
+// sides of a face see the same two traces either way, so the interface
+// flux stays single-valued (conservative).
+constexpr bool kOneX = OD_FAST;
 // Order of the BR1 traces (rhs.cu:385-455): x, y pairwise; z sequential from 0.
-__device__ __forceinline__ constexpr int br1_order(int d) { return d == 2 ? 1 : 0; }
+__device__ __forceinline__ constexpr int br1_order(int d) { return (d == 2 && !kOneX) ? 1 : 0; }
 // Order of the Riemann inputs and gradient slices (rhs.cu:126-180, 540-543,
 // 663-674, 716-727): x sequential from the first product, y pairwise, z from 0.
-// The fast build keeps a single x trace per face (faces 0/1, pairwise order)
-// for BR1 and the Riemann flux alike; the strict build keeps the reference's
-// two orders (0/1 pairwise for BR1, 6/7 sequential for the interface flux).
-// Both sides of a face still see the same two traces, so the interface flux
-// stays single-valued (conservative) either way.
-constexpr bool kOneX = OD_FAST;
-__device__ __forceinline__ constexpr int face_order(int d) { return d == 0 ? (kOneX ? 0 : 2) : (d == 1 ? 0 : 1); }
+__device__ __forceinline__ constexpr int face_order(int d) { return kOneX ? 0 : (d == 0 ? 2 : (d == 1 ? 0 : 1)); }
+// Faces of the trace layout the build writes, and the direction and order of face f.
+constexpr int kOwnFaces = kOneX ? 6 : 8;
+__device__ __forceinline__ int trace_face_dir(int f) { return f < 6 ? f >> 1 : 0; }
+__device__ __forceinline__ int trace_face_order(int f) { return kOneX ? 0 : (f < 4 ? 0 : (f < 6 ? 1 : 2)); }
 
 template <class C>
 struct Coef4 {
@@ -780,45 +785,32 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     }
 }
 
-// Face traces of the state tile -> tr_out.  Wave 0 writes the x traces
-// (strict build: pairwise and sequential orders, two passes; fast build:
-// the pairwise ones of both elements in one pass), wave 1 the y and z
-// traces of both elements in two passes.
+// Orders this wave's earlier LDS writes before its later LDS reads (of
+// other lanes' data): a wave's LDS operations execute in order, so this only
+// has to keep the compiler from moving them.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Face traces of one element (its state tile sUe in LDS) -> dst, the
+// element's kTraceStride block of tr_out.  The element's own wave writes
+// them: 16 lanes per face, 4 faces per pass.  Without BR1 (Euler) the strict
+// build skips the pairwise x traces, which only BR1 reads.
 template <class CT>
-__device__ __forceinline__ void write_traces2(const CT (*sU)[5][64], CT* tr_out, const Tile& tl, int wave, int lane,
-                                              bool need_pairwise_x) {
-    const int fsel = lane >> 4, fp = lane & 15;
-    if (kOneX && wave == 0) {
-        const int le = fsel >> 1, f = fsel & 1;
-        if (!tile_valid(tl, le)) return;
+__device__ __forceinline__ void write_traces_own(const CT (*sUe)[64], CT* dst, int lane, bool need_pairwise_x) {
+    const int fp = lane & 15;
+#pragma unroll
+    for (int pass = 0; pass < (kOwnFaces + 3) / 4; ++pass) {
+        const int f = 4 * pass + (lane >> 4);
+        if (f >= kOwnFaces || (!kOneX && !need_pairwise_x && f < 2)) continue;
         int base, stride;
-        face_line(0, fp, base, stride);
-        const Coef4<CT> c = side_coef<CT>(f);
-        CT* dst = tr_out + tile_eidx(tl, le) * kTraceStride + f * 80 + fp;
+        face_line(trace_face_dir(f), fp, base, stride);
+        const Coef4<CT> c = side_coef<CT>(f & 1);
+        const int order = trace_face_order(f);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) dst[v * 16] = trace4(sU[le][v], base, stride, c.c0, c.c1, c.c2, c.c3, 0);
-        return;
-    }
-    int f, d, order;
-    if (wave == 0) {
-        f = fsel < 2 ? fsel : fsel + 4;  // 0, 1 pairwise; 6, 7 sequential from the first product
-        d = 0;
-        order = fsel < 2 ? 0 : 2;
-    } else {
-        f = 2 + fsel;  // 2, 3 y pairwise; 4, 5 z sequential from 0
-        d = fsel < 2 ? 1 : 2;
-        order = fsel < 2 ? 0 : 1;
-    }
-    if (wave == 0 && !need_pairwise_x && fsel < 2) return;
-    int base, stride;
-    face_line(d, fp, base, stride);
-    const Coef4<CT> c = side_coef<CT>(f & 1);
-#pragma unroll
-    for (int le = 0; le < kEpb; ++le) {
-        if (!tile_valid(tl, le)) continue;
-        CT* dst = tr_out + tile_eidx(tl, le) * kTraceStride + f * 80 + fp;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) dst[v * 16] = trace4(sU[le][v], base, stride, c.c0, c.c1, c.c2, c.c3, order);
+        for (int v = 0; v < 5; ++v) dst[f * 80 + v * 16 + fp] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, order);
     }
 }
 
@@ -877,8 +869,8 @@ __device__ __forceinline__ void trace_body(const StageArgs<UT, CT>& a, const Til
 #pragma unroll
         for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
     }
-    __syncthreads();
-    write_traces2(sU, a.tr_out, tl, wave, pt, true);
+    __syncthreads();  // orders the coefficient table (filled by wave 0)
+    if (tile_valid(tl, wave)) write_traces_own(sU[wave], a.tr_out + tile_eidx(tl, wave) * kTraceStride, pt, true);
 }
 
 // Corrected gradients of the tile's elements and the viscous fluxes at
@@ -1054,12 +1046,13 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     }
     if (a.mode == 1) return;
 
-    // Face traces of the new state for the next stage.
-    __syncthreads();
+    // Face traces of the new state for the next stage, each wave those of its
+    // own element: from the RK phase on only wave w reads sU[w], so no
+    // work-group barrier is needed.
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)unew[v];
-    __syncthreads();
-    write_traces2(L.sU, a.tr_out, tl, wave, pt, VISC);
+    wave_sync();
+    if (mine) write_traces_own(L.sU[wave], a.tr_out + tile_eidx(tl, wave) * kTraceStride, pt, VISC);
 }
 
 template <class UT, class CT>
