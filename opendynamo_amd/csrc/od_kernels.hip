@@ -537,17 +537,22 @@ __device__ __forceinline__ int64_t tile_nb_rt(const Tile& t, int le, int f) {
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour-data prefetch (f32 compute).  At kernel start each work-group
-// copies everything it will read from its neighbours straight from global
-// memory into LDS with gfx950's global_load_lds_dwordx4 (no registers, one
-// exposed latency), as 16-byte chunks:
+// Neighbour-trace prefetch (f32 compute).  At kernel start each work-group
+// copies the neighbour traces it will read straight from global memory into
+// LDS with gfx950's global_load_lds_dwordx4 (no registers, one exposed
+// latency), as 16-byte chunks:
 //   A [0, 240)    neighbour facing traces, BR1 order, faces 0..5: 2 x 6 x 80
 //   B [240, 320)  neighbour facing x traces, sequential order:   2 x 2 x 80
-//   C [320, 512)  own face viscous fluxes fv:                    2 x 384
-//   D [512, 704)  neighbour facing fv:                           2 x 6 x 64
-// The float offsets of the four groups in the buffer:
-constexpr int kPfA = 0, kPfB = 960, kPfC = 1280, kPfD = 2048, kPfFloats = 2816;
-constexpr int kPfChunksVisc = 240, kPfChunksEuler = 320, kPfChunksFull = 704;
+//                 (strict build only; the fast build has one x trace)
+// The face viscous fluxes fv are read from global memory in the face pass
+// instead: staging them too (another 1536 floats) limited the update kernel
+// to 4.5 waves per SIMD by LDS; without them it runs 6 (a step measured
+// ~6 % faster in both builds).
+// The float offsets of the two groups in the buffer:
+constexpr int kPfA = 0, kPfB = 960;
+constexpr int kPfChunksVisc = 240;                            // visc kernel: A
+constexpr int kPfChunksUpdate = kOneX ? 240 : 320;            // update kernel: A (+ B)
+constexpr int kPfFloatsUpdate = 4 * kPfChunksUpdate;
 
 // Source of chunk `id`; false for chunks of a missing second element (odd
 // nx), which are not copied.
@@ -558,21 +563,12 @@ __device__ __forceinline__ bool prefetch_chunk(const StageArgs<UT, CT>& a, const
         if (!tile_valid(tl, le)) return false;
         const int64_t own = tile_eidx(tl, le), nb = tile_nb_rt(tl, le, f);
         src = a.tr_in + (nb >= 0 ? nb : own) * kTraceStride + (f ^ 1) * 80 + c * 4;
-    } else if (id < 320) {  // B
+    } else {  // B
         if (kOneX) return false;  // the x Riemann inputs are group A's
         const int r = id - 240, le = r / 40, f = (r % 40) / 20, c = r % 20;
         if (!tile_valid(tl, le)) return false;
         const int64_t nb = tile_nb_rt(tl, le, f);  // x is periodic: always a neighbour
         src = a.tr_in + nb * kTraceStride + (f == 0 ? 7 : 6) * 80 + c * 4;
-    } else if (id < 512) {  // C
-        const int r = id - 320, le = r / 96, c = r % 96;
-        if (!tile_valid(tl, le)) return false;
-        src = a.fv + tile_eidx(tl, le) * kFvStride + c * 4;
-    } else {  // D
-        const int r = id - 512, le = r / 96, f = (r % 96) / 16, c = r % 16;
-        if (!tile_valid(tl, le)) return false;
-        const int64_t own = tile_eidx(tl, le), nb = tile_nb_rt(tl, le, f);
-        src = a.fv + (nb >= 0 ? nb : own) * kFvStride + (f ^ 1) * 64 + c * 4;
     }
     return true;
 }
@@ -728,7 +724,7 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     CT own[5], oth[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) own[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
-    const int64_t nb = PF ? 0 : tile_nb<D>(tl, t.le, t.side);
+    const int64_t nb = (PF && !VISC) ? 0 : tile_nb<D>(tl, t.le, t.side);
     const bool wall = PF ? tile_wall<D>(tl, t.le, t.side) : nb < 0;
     // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
     const int rf = (D == 0 && !kOneX) ? (right ? 6 : 7) : (t.f ^ 1);
@@ -746,14 +742,8 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     if (VISC) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            if constexpr (PF) {
-                fvo[1 + q] = pf[kPfC + t.le * kFvStride + t.f * 64 + q * 16 + t.fp];
-                const CT far = pf[kPfD + (t.le * 6 + t.f) * 64 + q * 16 + t.fp];
-                fvn[1 + q] = wall ? fvo[1 + q] : far;
-            } else {
-                fvo[1 + q] = a.fv[tile_eidx(tl, t.le) * kFvStride + t.f * 64 + q * 16 + t.fp];
-                fvn[1 + q] = wall ? fvo[1 + q] : a.fv[nb * kFvStride + (t.f ^ 1) * 64 + q * 16 + t.fp];
-            }
+            fvo[1 + q] = a.fv[tile_eidx(tl, t.le) * kFvStride + t.f * 64 + q * 16 + t.fp];
+            fvn[1 + q] = wall ? fvo[1 + q] : a.fv[nb * kFvStride + (t.f ^ 1) * 64 + q * 16 + t.fp];
         }
     }
     CT L[5], R[5];
@@ -845,7 +835,7 @@ struct ViscLds {
 template <class CT, bool VISC>
 struct UpdateLds {
     static constexpr bool kPF = std::is_same<CT, float>::value;
-    float pf[kPF ? (VISC ? kPfFloats : kPfC) : 4];
+    float pf[kPF ? kPfFloatsUpdate : 4];
     CT sU[kEpb][5][64];
     CT sJmp[kEpb][6][5][16];
     CT sJ_own[kPF ? 0 : kEpb][6][5][16];  // zero-length when aliased
@@ -917,7 +907,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     const bool mine = tile_valid(tl, wave);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t gbase = tile_eidx(tl, wave) * 64 + pt;
-    if constexpr (PF) prefetch_issue(a, tl, L.pf, VISC ? kPfChunksFull : kPfChunksEuler, wave, pt);
+    if constexpr (PF) prefetch_issue(a, tl, L.pf, kPfChunksUpdate, wave, pt);
 
     constexpr bool SAME = std::is_same<UT, CT>::value;  // sU holds the state exactly
     UT ucur[5];
