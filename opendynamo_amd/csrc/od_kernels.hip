@@ -1059,14 +1059,19 @@ __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
     visc_body(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
-// f32 compute: ask for 4 waves per SIMD (128 VGPRs; a few spill, measured
-// faster than 3 waves without); f64 compute needs ~270 VGPRs and keeps its
-// natural occupancy.
+// Occupancy target of the update kernels.  f32 compute needs ~73-78 VGPRs
+// (6 waves per SIMD); the fast all-f32 single-box kernel is held to 72 (7
+// waves, 2 registers spill; measured ~1 % faster per step).  The strict
+// build, f64 states and the batch kernels spill too much at 72 and keep
+// their natural allocation (the 4-wave cap does not bind); f64 compute
+// needs ~145 VGPRs.
 template <class CT>
 constexpr int kMinWaves = sizeof(CT) == 4 ? 4 : 1;
+template <class UT, class CT>
+constexpr int kMinWavesUpdate = (OD_FAST && sizeof(UT) == 4 && sizeof(CT) == 4) ? 7 : kMinWaves<CT>;
 
 template <class UT, class CT, bool VISC>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWaves<CT>))) void od_update_kernel(
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWavesUpdate<UT, CT>))) void od_update_kernel(
     StageArgs<UT, CT> a) {
     __shared__ UpdateLds<CT, VISC> L;
     coef_fill<CT>(threadIdx.x);
