@@ -392,6 +392,23 @@ __device__ __forceinline__ C trace4(const C* q, int base, int stride, C c0, C c1
     return t0 + t1 + t2 + t3;
 }
 
+// Bank-conflict-free position of point p = i + 4j + 16k in a 64-value LDS
+// row: a face pass reads, per half-wave, the 16 face lines of one element at
+// one line position (fixed i, j or k), which in the plain layout hit 8 of
+// the 32 banks twice for x and y faces.  Bits of the position: i0^j0, i0^k1,
+// i1, j1, k0, i0 -- a bijection under which each of those 16-point sets, and
+// the 32 points a half-wave writes, fall in 16 or 32 distinct banks.
+__device__ __forceinline__ int swz64(int p) {
+    const int i0 = p & 1, j0 = (p >> 2) & 1, k1 = (p >> 5) & 1;
+    return (i0 ^ j0) | ((i0 ^ k1) << 1) | (p & 2) << 1 | (p & 8) | (p & 16) | (i0 << 5);
+}
+// trace4 over a swizzled row: line points base + m * stride.
+template <class C>
+__device__ __forceinline__ C trace4_swz(const C* q, int base, int stride, C c0, C c1, C c2, C c3, int order) {
+    const C r[4] = {q[swz64(base)], q[swz64(base + stride)], q[swz64(base + 2 * stride)], q[swz64(base + 3 * stride)]};
+    return trace4(r, 0, 1, c0, c1, c2, c3, order);
+}
+
 // The fast build forms every trace in the pairwise (fused) form and keeps a
 // single x trace per face (faces 0/1) for BR1 and the Riemann flux alike;
 // the strict build keeps the reference's orders below, with two x traces
@@ -701,7 +718,7 @@ __device__ __forceinline__ void fv_pass(const StageArgs<UT, CT>& a, const CT (*s
 #pragma unroll
     for (int v = 0; v < 5; ++v) st[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
 #pragma unroll
-    for (int q = 0; q < 15; ++q) gt[q] = trace4(sG[t.le][q], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+    for (int q = 0; q < 15; ++q) gt[q] = trace4_swz(sG[t.le][q], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
     CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
     if (a.mu_t) mu = mu + a.mu_t[tile_eidx(tl, t.le)];
     viscous_flux(D, st, gt, mu, fvv);
@@ -885,7 +902,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<UT, CT>& a, const Tile
     if constexpr (PF) __syncthreads();  // sG overwrites the jumps the other wave may still read
     if (tile_valid(tl, wave)) {
 #pragma unroll
-        for (int q = 0; q < 15; ++q) L.sG()[wave][q][pt] = g[q];
+        for (int q = 0; q < 15; ++q) L.sG()[wave][q][swz64(pt)] = g[q];
     }
     __syncthreads();
     if (wave == 0) {
