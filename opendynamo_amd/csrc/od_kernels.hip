@@ -115,8 +115,7 @@ __device__ __forceinline__ void euler_flux(int d, const C* s, C* f) {  // :7-21
 // a point: velocity, stress tensor and the heat flux along each direction.
 // Every entry is the expression the directional flux evaluates, so the
 // fluxes assembled from it are bitwise those of compute_viscous_flux_{x,y,z};
-// computing it once frees the 15 gradient values before the three
-// directions are assembled (fewer live registers in the volume pass).
+// the volume pass forms it once for its three directions.
 template <class C>
 struct ViscPoint {
     C u[3];
@@ -393,9 +392,10 @@ __device__ __forceinline__ C trace4(const C* q, int base, int stride, C c0, C c1
 }
 
 // Bank-conflict-free position of point p = i + 4j + 16k in a 64-value LDS
-// row: a face pass reads, per half-wave, the 16 face lines of one element at
-// one line position (fixed i, j or k), which in the plain layout hit 8 of
-// the 32 banks twice for x and y faces.  Bits of the position: i0^j0, i0^k1,
+// row (the visc kernel's gradient tile is stored in this order): a face pass
+// reads, per half-wave, the 16 face lines of one element at one line
+// position (fixed i, j or k), which in the plain layout hit 8 of the 32
+// banks twice for x and y faces.  Bits of the position: i0^j0, i0^k1,
 // i1, j1, k0, i0 -- a bijection under which each of those 16-point sets, and
 // the 32 points a half-wave writes, fall in 16 or 32 distinct banks.
 __device__ __forceinline__ int swz64(int p) {
@@ -743,7 +743,8 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     for (int v = 0; v < 5; ++v) own[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
     const int64_t nb = (PF && !VISC) ? 0 : tile_nb<D>(tl, t.le, t.side);
     const bool wall = PF ? tile_wall<D>(tl, t.le, t.side) : nb < 0;
-    // the neighbour's facing trace in the same order (x: 7/6, y: 3/2, z: 5/4)
+    // the neighbour's facing trace in the same order (x: 7/6 strict, 1/0 fast;
+    // y: 3/2; z: 5/4)
     const int rf = (D == 0 && !kOneX) ? (right ? 6 : 7) : (t.f ^ 1);
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
