@@ -1019,9 +1019,15 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     CT r[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
-        uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
         ucr[v] = SAME ? (UT)L.sU[wave][v][pt] : ucur[v];
         r[v] = L.sF(wave)[v][pt];
+    }
+    if (a.stage1) {  // u^n is the state this stage read (uprev == uin): no second load
+#pragma unroll
+        for (int v = 0; v < 5; ++v) uprv[v] = ucr[v];
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
     }
     UT unew[5];
 #pragma unroll
