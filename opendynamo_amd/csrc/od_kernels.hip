@@ -1084,11 +1084,11 @@ __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
 }
 
 // Occupancy target of the update kernels.  f32 compute needs ~73-78 VGPRs
-// (6 waves per SIMD); the fast all-f32 single-box kernel is held to 72 (7
-// waves, 2 registers spill; measured ~1 % faster per step).  The strict
-// build, f64 states and the batch kernels spill too much at 72 and keep
-// their natural allocation (the 4-wave cap does not bind); f64 compute
-// needs ~145 VGPRs.
+// (6 waves per SIMD); the fast all-f32 single-box kernel and the batch
+// kernel's single-unit form are held to 72 (7 waves, a few registers spill;
+// measured ~1 % faster per step).  The strict build, f64 states and the
+// persistent batch kernel spill too much at 72 and keep their natural
+// allocation (the 4-wave cap does not bind); f64 compute needs ~145 VGPRs.
 template <class CT>
 constexpr int kMinWaves = sizeof(CT) == 4 ? 4 : 1;
 template <class UT, class CT>
@@ -1200,13 +1200,16 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
     tl.v1 = ok && tl.v1;
 }
 
-template <class UT, class CT, int PAIRS>
+// LOOP = false: one unit per work-group (grid >= units, order 0); the
+// persistent loop is a separate instantiation, so that its loop-carried
+// state does not raise the register count of the single-unit kernel.
+template <class UT, class CT, int PAIRS, bool LOOP>
 __global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ ViscLds<CT> L[PAIRS];
     coef_fill<CT>(threadIdx.x);
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
-    if ((int)gridDim.x >= b.n_units && b.order == 0) {  // one unit per work-group: no loop
+    if constexpr (!LOOP) {
         StageArgs<UT, CT> ab;
         Tile tl;
         batch_tile(a, b, (int)blockIdx.x * PAIRS + pair, ab, tl);
@@ -1229,14 +1232,14 @@ __global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT
     }
 }
 
-template <class UT, class CT, bool VISC, int PAIRS>
-__global__ __launch_bounds__(128 * PAIRS) __attribute__((amdgpu_waves_per_eu(kMinWaves<CT>))) void od_batch_update_kernel(
+template <class UT, class CT, bool VISC, int PAIRS, bool LOOP>
+__global__ __launch_bounds__(128 * PAIRS) __attribute__((amdgpu_waves_per_eu(LOOP ? kMinWaves<CT> : kMinWavesUpdate<UT, CT>))) void od_batch_update_kernel(
     StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ UpdateLds<CT, VISC> L[PAIRS];
     coef_fill<CT>(threadIdx.x);
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
-    if ((int)gridDim.x >= b.n_units && b.order == 0) {  // one unit per work-group: no loop
+    if constexpr (!LOOP) {
         StageArgs<UT, CT> ab;
         Tile tl;
         batch_tile(a, b, (int)blockIdx.x * PAIRS + pair, ab, tl);
@@ -1323,21 +1326,37 @@ int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
 
 template <class UT, class CT>
 int launch_batch_visc(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream) {
-    if (pairs == 2)
-        hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 2>), dim3(groups), dim3(256), 0, (hipStream_t)stream, a, b);
-    else
-        hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 1>), dim3(groups), dim3(128), 0, (hipStream_t)stream, a, b);
+    const bool single = groups >= b.n_units && b.order == 0;  // one unit per work-group
+    const hipStream_t st = (hipStream_t)stream;
+    if (pairs == 2) {
+        if (single)
+            hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 2, false>), dim3(groups), dim3(256), 0, st, a, b);
+        else
+            hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 2, true>), dim3(groups), dim3(256), 0, st, a, b);
+    } else {
+        if (single)
+            hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 1, false>), dim3(groups), dim3(128), 0, st, a, b);
+        else
+            hipLaunchKernelGGL((od_batch_visc_kernel<UT, CT, 1, true>), dim3(groups), dim3(128), 0, st, a, b);
+    }
     return check_launch();
 }
 
 template <class UT, class CT, bool VISC>
 int launch_batch_update(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream) {
-    if (pairs == 2)
-        hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 2>), dim3(groups), dim3(256), 0, (hipStream_t)stream,
-                           a, b);
-    else
-        hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 1>), dim3(groups), dim3(128), 0, (hipStream_t)stream,
-                           a, b);
+    const bool single = groups >= b.n_units && b.order == 0;  // one unit per work-group
+    const hipStream_t st = (hipStream_t)stream;
+    if (pairs == 2) {
+        if (single)
+            hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 2, false>), dim3(groups), dim3(256), 0, st, a, b);
+        else
+            hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 2, true>), dim3(groups), dim3(256), 0, st, a, b);
+    } else {
+        if (single)
+            hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 1, false>), dim3(groups), dim3(128), 0, st, a, b);
+        else
+            hipLaunchKernelGGL((od_batch_update_kernel<UT, CT, VISC, 1, true>), dim3(groups), dim3(128), 0, st, a, b);
+    }
     return check_launch();
 }
 
@@ -1345,17 +1364,20 @@ template <class UT, class CT>
 int batch_occupancy(int kind, int pairs, bool visc, int* groups_per_cu) {
     int n = 0;
     hipError_t e;
+    // the persistent (loop) kernels: the grid this sizes is a persistent one
     if (kind == 0)
-        e = pairs == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_visc_kernel<UT, CT, 2>, 256, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_visc_kernel<UT, CT, 1>, 128, 0);
+        e = pairs == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_visc_kernel<UT, CT, 2, true>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_visc_kernel<UT, CT, 1, true>, 128, 0);
     else if (visc)
-        e = pairs == 2
-                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, true, 2>, 256, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, true, 1>, 128, 0);
+        e = pairs == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &n, od_batch_update_kernel<UT, CT, true, 2, true>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &n, od_batch_update_kernel<UT, CT, true, 1, true>, 128, 0);
     else
-        e = pairs == 2
-                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, false, 2>, 256, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, od_batch_update_kernel<UT, CT, false, 1>, 128, 0);
+        e = pairs == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &n, od_batch_update_kernel<UT, CT, false, 2, true>, 256, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &n, od_batch_update_kernel<UT, CT, false, 1, true>, 128, 0);
     *groups_per_cu = n;
     return e == hipSuccess ? 0 : 1;
 }
