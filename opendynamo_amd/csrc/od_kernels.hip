@@ -446,15 +446,20 @@ __device__ __forceinline__ CoefTable<C>& coef_lds() {
     __shared__ CoefTable<C> t;
     return t;
 }
-// Threads 0..31 of the work-group copy the table from constant memory; the
-// kernel's first barrier orders the copy before every read.
+// Threads 0..31 of the work-group copy the table from constant memory: the
+// load is issued first thing (coef_load), the LDS store comes just before the
+// body's first barrier (coef_store), which orders it before every read, so
+// the load's latency hides behind the body's own loads.
 template <class C>
-__device__ __forceinline__ void coef_fill(int tid) {
+__device__ __forceinline__ C coef_load(int tid) {
     static_assert(sizeof(Ops<C>) == 32 * sizeof(C) && sizeof(CoefTable<C>) == 32 * sizeof(C), "table layout");
-    if (tid >= 32) return;
     // Ops order: D[16], eL[4], eR[4], gL[4], gR[4]
     const int src = tid < 16 ? tid : (tid < 24 ? 24 + ((tid - 16) & 1) * 4 + ((tid - 16) >> 1) : tid - 8);
-    reinterpret_cast<C*>(&coef_lds<C>())[tid] = reinterpret_cast<const C*>(&dev_ops<C>())[src];
+    return tid < 32 ? reinterpret_cast<const C*>(&dev_ops<C>())[src] : (C)0;
+}
+template <class C>
+__device__ __forceinline__ void coef_store(int tid, C v) {
+    if (tid < 32) reinterpret_cast<C*>(&coef_lds<C>())[tid] = v;
 }
 template <class C>
 __device__ __forceinline__ Coef4<C> side_coef(bool right) {
@@ -872,12 +877,14 @@ struct UpdateLds {
 template <class UT, class CT>
 __device__ __forceinline__ void trace_body(const StageArgs<UT, CT>& a, const Tile& tl, CT (*sU)[5][64], int wave,
                                            int pt) {
+    const CT cf = coef_load<CT>((int)threadIdx.x);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     if (tile_valid(tl, wave)) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
     }
-    __syncthreads();  // orders the coefficient table (filled by wave 0)
+    coef_store<CT>((int)threadIdx.x, cf);
+    __syncthreads();
     if (tile_valid(tl, wave)) write_traces_own(sU[wave], a.tr_out + tile_eidx(tl, wave) * kTraceStride, pt, true);
 }
 
@@ -889,11 +896,13 @@ __device__ __forceinline__ void visc_body(const StageArgs<UT, CT>& a, const Tile
     constexpr bool PF = std::is_same<CT, float>::value;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    const CT cf = coef_load<CT>((int)threadIdx.x);
     if constexpr (PF) prefetch_issue(a, tl, L.pf, kPfChunksVisc, wave, pt);
     if (tile_valid(tl, wave)) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
     }
+    coef_store<CT>((int)threadIdx.x, cf);
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
     run_br1_jumps<PF>(a, L.sU, L.sJmp, tl, reinterpret_cast<const CT*>(L.pf), wave, pt);
@@ -925,6 +934,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     const bool mine = tile_valid(tl, wave);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t gbase = tile_eidx(tl, wave) * 64 + pt;
+    const CT cf = coef_load<CT>((int)threadIdx.x);
     if constexpr (PF) prefetch_issue(a, tl, L.pf, kPfChunksUpdate, wave, pt);
 
     constexpr bool SAME = std::is_same<UT, CT>::value;  // sU holds the state exactly
@@ -934,6 +944,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
         ucur[v] = mine ? a.uin[v * NP + gbase] : (UT)1;
         L.sU[wave][v][pt] = (CT)ucur[v];
     }
+    coef_store<CT>((int)threadIdx.x, cf);
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
 
@@ -1072,14 +1083,12 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 template <class UT, class CT>
 __global__ __launch_bounds__(128) void od_trace_kernel(StageArgs<UT, CT> a) {
     __shared__ CT sU[kEpb][5][64];
-    coef_fill<CT>(threadIdx.x);
     trace_body(a, make_tile(a), sU, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
 template <class UT, class CT>
 __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
     __shared__ ViscLds<CT> L;
-    coef_fill<CT>(threadIdx.x);
     visc_body(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
@@ -1098,7 +1107,6 @@ template <class UT, class CT, bool VISC>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWavesUpdate<UT, CT>))) void od_update_kernel(
     StageArgs<UT, CT> a) {
     __shared__ UpdateLds<CT, VISC> L;
-    coef_fill<CT>(threadIdx.x);
     update_body<UT, CT, VISC>(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
@@ -1206,7 +1214,6 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
 template <class UT, class CT, int PAIRS, bool LOOP>
 __global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ ViscLds<CT> L[PAIRS];
-    coef_fill<CT>(threadIdx.x);
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
     if constexpr (!LOOP) {
@@ -1236,7 +1243,6 @@ template <class UT, class CT, bool VISC, int PAIRS, bool LOOP>
 __global__ __launch_bounds__(128 * PAIRS) __attribute__((amdgpu_waves_per_eu(LOOP ? kMinWaves<CT> : kMinWavesUpdate<UT, CT>))) void od_batch_update_kernel(
     StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ UpdateLds<CT, VISC> L[PAIRS];
-    coef_fill<CT>(threadIdx.x);
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
     if constexpr (!LOOP) {
