@@ -281,10 +281,15 @@ def main():
     upd_ms = kern["od_update_kernel"]
     alg_bytes_launch = ALG_BYTES_PER_DOF_STEP / 3.0 * dof
     achieved = alg_bytes_launch / (upd_ms / 1e3) / 1e9
+    traffic = traffic_from_profile()
     roofline = {
         "bound": "hbm", "kernel": "od_update_kernel", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_from_profile(),
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
         "algorithmic_bytes_per_launch": alg_bytes_launch, "launch_ms": round(upd_ms, 5),
+        # the profiled HBM bytes per launch over the live launch time: how much
+        # of the peak the kernel actually moves (the design's face traces and
+        # viscous face fluxes on top of the compulsory state traffic)
+        "traffic_frac": round(traffic / (upd_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5) if traffic else None,
         "step_frac_alg_hbm": round(value * 1e9 * ALG_BYTES_PER_DOF_STEP / 1e9 / HBM_PEAK_GBS, 5),
         "valu_frac_alg_flops": round(value * FLOP_PER_DOF_STEP / 1e3 / FP32_VECTOR_PEAK_TFLOPS, 5),
     }
