@@ -17,8 +17,13 @@ the spectra, which are numpy/torch post-processing):
      kernel the closure evaluates at run time);
   5. training: od_mlp_train (BF16 MFMA forward/backward, Adam) on the
      standardised features, with the training range kept for the fallback;
-  6. evaluation: held-out seeds, energy spectra of coarse+closure, plain
-     coarse and plain intermediate runs against the fine run.
+  6. a-posteriori calibration: one multiplier of the network's output
+     scale, chosen on calibration seeds (from the training seeds) as the
+     one with the smallest spectral error at the evaluation time among
+     those whose coarse runs all stay finite to the run time;
+  7. evaluation: held-out seeds, energy spectra of coarse+closure, plain
+     coarse and plain intermediate runs against the fine run, at every
+     check time the runs stay finite.
 """
 import argparse
 import json
@@ -239,7 +244,7 @@ def spectral_error(E, E_ref, k_max):
 DEFAULT = dict(L=TWO_PI, mach=0.1, re=400.0, n_modes=64, amplitude=0.1, kmax=4, flux="slau2", fine=48, ratio=3,
                mid=32, snap_times=[5.0, 10.0, 15.0, 20.0, 25.0, 30.0, 35.0, 40.0], eval_time=5.0, run_time=40.0,
                train_seeds=list(range(8)), eval_seeds=[100, 101], epochs=300, batch=65536, lr=2e-3,
-               update_interval=5, budget_kappa=1.0)
+               update_interval=5, budget_kappa=1.0, calib_seeds=[0, 1], multipliers=[1, 2, 4, 8, 16, 32])
 
 
 def viscosity(cfg):
@@ -304,37 +309,83 @@ def train(cfg, X, T, device=0, log=print):
                    r2_train=r2_train, r2_heldout=r2_val)
 
 
+def check_times(cfg):
+    return [5.0 * q for q in range(1, int(cfg["run_time"] / 5.0) + 1)]
+
+
+def fine_spectra(cfg, seed, times):
+    nf = cfg["fine"]
+    fine = run(nf, cfg["L"], viscosity(cfg), seed, cfg, times)
+    return [energy_spectrum(uniform_velocity(st, nf), cfg["L"]) for st in fine["states"]], fine
+
+
+def coarse_errors(cfg, n, seed, times, E_refs, k_max, mlp=None):
+    """Run n^3 from the seed's IC to each of `times`; spectral error at each
+    time the state is finite (None after it diverges) and the GPU wall time."""
+    res = run(n, cfg["L"], viscosity(cfg), seed, cfg, times, mlp=mlp, update_interval=cfg["update_interval"],
+              budget_kappa=cfg["budget_kappa"])
+    errs, specs = [], []
+    for st, E_ref in zip(res["states"], E_refs):
+        if np.isfinite(st).all():
+            E = energy_spectrum(uniform_velocity(st, n), cfg["L"])
+            errs.append(spectral_error(E, E_ref, k_max))
+            specs.append(E[:k_max].tolist())
+        else:
+            errs.append(None)
+            specs.append(None)
+    fin = [e is not None for e in errs]
+    finite_until = max([t for t, f in zip(times, fin) if f], default=0.0)
+    return dict(errs=errs, spectra=specs, finite_until=finite_until, wall_s=res["wall"], steps=res["steps"],
+                stats=res["stats"])
+
+
+def calibrate(cfg, mlp, base_scale, log=print):
+    """A-posteriori multiplier of the output scale (see the module docstring).
+    Returns the chosen multiplier and the trial table."""
+    nc = cfg["fine"] // cfg["ratio"]
+    k_max = (nc * 4) // 3
+    times = check_times(cfg)
+    te = times.index(cfg["eval_time"])
+    refs = {seed: fine_spectra(cfg, seed, times)[0] for seed in cfg["calib_seeds"]}
+    table = []
+    for m in cfg["multipliers"]:
+        mlp.set_output_scale(base_scale * m)
+        rows = [coarse_errors(cfg, nc, seed, times, refs[seed], k_max, mlp) for seed in cfg["calib_seeds"]]
+        stable = all(r["finite_until"] >= cfg["run_time"] for r in rows)
+        e = [r["errs"][te] for r in rows]
+        err = float(np.mean(e)) if all(x is not None for x in e) else None
+        table.append(dict(multiplier=m, err_eval_time=err, stable=stable,
+                          finite_until=[r["finite_until"] for r in rows]))
+        log(f"calibration x{m}: mean spectral error at t={cfg['eval_time']} {err}, finite until "
+            f"{[r['finite_until'] for r in rows]}")
+    ok = [t for t in table if t["stable"] and t["err_eval_time"] is not None]
+    if ok:
+        best = min(ok, key=lambda t: t["err_eval_time"])
+    else:  # none stays finite: the most stable, then the most accurate
+        best = max(table, key=lambda t: (min(t["finite_until"]), -(t["err_eval_time"] or 1e30)))
+    mlp.set_output_scale(base_scale * best["multiplier"])
+    return best["multiplier"], table
+
+
 def evaluate(cfg, mlp, log=print):
-    mu = viscosity(cfg)
     nf, r, nm = cfg["fine"], cfg["ratio"], cfg["mid"]
     nc = nf // r
     k_max = (nc * 4) // 3  # 2/3 of the coarse grid's Nyquist wavenumber (4 nc / 2)
+    times = check_times(cfg)
+    te = times.index(cfg["eval_time"])
     rows = []
-    te = cfg["eval_time"]
     for seed in cfg["eval_seeds"]:
-        fine = run(nf, cfg["L"], mu, seed, cfg, [te])
-        E_ref = energy_spectrum(uniform_velocity(fine["states"][-1], nf), cfg["L"])
-        out = {"seed": seed, "fine_wall_s": fine["wall"], "fine_steps": fine["steps"]}
+        E_refs, fine = fine_spectra(cfg, seed, times)
+        out = {"seed": seed, "times": times, "fine_wall_s": fine["wall"], "fine_steps": fine["steps"]}
         for name, n, net in (("coarse_closure", nc, mlp), ("coarse_plain", nc, None), ("mid_plain", nm, None)):
-            res = run(n, cfg["L"], mu, seed, cfg, [te], mlp=net, update_interval=cfg["update_interval"],
-                      budget_kappa=cfg["budget_kappa"])
-            row = dict(wall_s=res["wall"], steps=res["steps"], stats=res["stats"])
-            if np.isfinite(res["states"][-1]).all():
-                E = energy_spectrum(uniform_velocity(res["states"][-1], n), cfg["L"])
-                row.update(err=spectral_error(E, E_ref, k_max), spectrum=E[:k_max].tolist())
-            else:
-                row.update(err=None, spectrum=None)
-            # divergence time of the same run continued to t_run (checks every 5)
-            checks = [5.0 * q for q in range(1, int(cfg["run_time"] / 5.0) + 1)]
-            long = run(n, cfg["L"], mu, seed, cfg, checks, mlp=net, update_interval=cfg["update_interval"],
-                       budget_kappa=cfg["budget_kappa"])
-            fin = [bool(np.isfinite(st).all()) for st in long["states"]]
-            row["finite_until"] = max([t for t, f in zip(checks, fin) if f], default=0.0)
-            row["long_stats"] = long["stats"]
+            row = coarse_errors(cfg, n, seed, times, E_refs, k_max, net)
+            row["err"] = row["errs"][te]  # the evaluation time
+            row["spectrum"] = row["spectra"][te]
+            del row["spectra"]
             out[name] = row
-            log(f"seed {seed} {name}: spectral error at t={te} {row['err']}, {res['wall']:.2f} s, finite until "
-                f"t={row['finite_until']} (of {cfg['run_time']}), stats {res['stats']}")
-        out["fine_spectrum"] = E_ref[:k_max].tolist()
+            log(f"seed {seed} {name}: spectral error at t={times} {row['errs']}, {row['wall_s']:.2f} s, "
+                f"finite until t={row['finite_until']}, stats {row['stats']}")
+        out["fine_spectrum"] = E_refs[te][:k_max].tolist()
         rows.append(out)
     return dict(k_max=k_max, runs=rows)
 
@@ -353,6 +404,8 @@ def main(argv=None):
     ap.add_argument("--dataset", default=None, help="skip the fine runs: load features/targets (.npz)")
     ap.add_argument("--flux", default=DEFAULT["flux"])
     ap.add_argument("--kmax", type=int, default=DEFAULT["kmax"])
+    ap.add_argument("--multiplier", type=float, default=None,
+                    help="output-scale multiplier (default: calibrated a posteriori on the calibration seeds)")
     args = ap.parse_args(argv)
     cfg = dict(DEFAULT, fine=args.fine, ratio=args.ratio, mid=args.mid, train_seeds=list(range(args.seeds)),
                epochs=args.epochs, snap_times=args.snap_times, eval_time=args.eval_time, flux=args.flux,
@@ -364,7 +417,12 @@ def main(argv=None):
         w = np.load(args.weights)
         mlp = od.MLP(params=w["params"])
         mlp.set_scaling(w["shift"], w["scale"], w["lo"], w["hi"])
-        mlp.set_output_scale(float(w["out_scale"]))
+        base = float(w["out_scale"])
+        wts = {k: w[k] for k in ("params", "shift", "scale", "lo", "hi", "out_scale")}
+        prev = os.path.join(os.path.dirname(os.path.abspath(args.weights)), "closure_report.json")
+        if os.path.exists(prev):  # keep the training record of the weights
+            with open(prev) as f:
+                report["training"] = json.load(f).get("training")
     else:
         if args.dataset:
             d = np.load(args.dataset)
@@ -375,9 +433,19 @@ def main(argv=None):
             report["dataset_s"] = time.perf_counter() - t0
             np.savez(os.path.join(args.out, "closure_dataset.npz"), X=X, T=T)
         mlp, info = train(cfg, X, T, log=log)
-        np.savez(os.path.join(args.out, "closure_weights.npz"), params=mlp.get_params(), shift=info["shift"],
-                 scale=info["scale"], lo=info["lo"], hi=info["hi"], out_scale=info["out_scale"])
+        base = info["out_scale"]
+        wts = dict(params=mlp.get_params(), shift=info["shift"], scale=info["scale"], lo=info["lo"], hi=info["hi"],
+                   out_scale=base)
         report["training"] = {k: (v.tolist() if isinstance(v, np.ndarray) else v) for k, v in info.items()}
+    if args.multiplier is None:
+        mult, table = calibrate(cfg, mlp, base, log)
+        report["calibration"] = dict(multiplier=mult, trials=table)
+    else:
+        mult = args.multiplier
+        mlp.set_output_scale(base * mult)
+        report["calibration"] = dict(multiplier=mult, trials=None)
+    # out_scale is the trained scale; the closure runs with out_scale * output_multiplier
+    np.savez(os.path.join(args.out, "closure_weights.npz"), output_multiplier=mult, **wts)
     report["evaluation"] = evaluate(cfg, mlp, log)
     with open(os.path.join(args.out, "closure_report.json"), "w") as f:
         json.dump(report, f)

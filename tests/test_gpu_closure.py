@@ -240,9 +240,10 @@ def test_closure_budget_fallback_reverts_to_plain_fr():
 
 
 def test_committed_closure_model_runs_on_the_coarse_mesh():
-    """The trained network shipped with the package (48^3 -> 16^3, Re 400)
-    loads, attaches and produces a positive eddy viscosity on its own coarse
-    workload; the first 50 steps stay finite."""
+    """The trained network shipped with the package (48^3 -> 16^3, Re 400),
+    at its calibrated output scale, loads, attaches and produces a positive
+    eddy viscosity on its own coarse workload; the first 50 steps stay
+    finite."""
     import os
     from opendynamo_amd import closure as C
     w = np.load(os.path.join(os.path.dirname(od.__file__), "closure_model", "closure_weights.npz"))
@@ -253,7 +254,7 @@ def test_committed_closure_model_runs_on_the_coarse_mesh():
     u0 = C.initial_state(n, cfg["L"], 100, cfg["n_modes"], cfg["amplitude"], cfg["kmax"])
     with od.MLP(params=w["params"]) as m, od.Solver(p) as s:
         m.set_scaling(w["shift"], w["scale"], w["lo"], w["hi"])
-        m.set_output_scale(float(w["out_scale"]))
+        m.set_output_scale(float(w["out_scale"]) * float(w["output_multiplier"]))
         s.set_state(u0)
         dt = s.cfl_dt(0.3)
         s.attach_closure(m, update_interval=5, budget_kappa=1.0)
