@@ -453,8 +453,10 @@ __device__ __forceinline__ CoefTable<C>& coef_lds() {
 template <class C>
 __device__ __forceinline__ C coef_load(int tid) {
     static_assert(sizeof(Ops<C>) == 32 * sizeof(C) && sizeof(CoefTable<C>) == 32 * sizeof(C), "table layout");
-    // Ops order: D[16], eL[4], eR[4], gL[4], gR[4]
-    const int src = tid < 16 ? tid : (tid < 24 ? 24 + ((tid - 16) & 1) * 4 + ((tid - 16) >> 1) : tid - 8);
+    // Ops order: D[16], eL[4], eR[4], gL[4], gR[4]; the index stays inside
+    // Ops for every thread, whether or not the load is speculated
+    const int t = tid & 31;
+    const int src = t < 16 ? t : (t < 24 ? 24 + ((t - 16) & 1) * 4 + ((t - 16) >> 1) : t - 8);
     return tid < 32 ? reinterpret_cast<const C*>(&dev_ops<C>())[src] : (C)0;
 }
 template <class C>
