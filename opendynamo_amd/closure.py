@@ -19,8 +19,9 @@ the spectra, which are numpy/torch post-processing):
      standardised features, with the training range kept for the fallback;
   6. a-posteriori calibration: one multiplier of the network's output
      scale, chosen on calibration seeds (from the training seeds) as the
-     one with the smallest spectral error at the evaluation time among
-     those whose coarse runs all stay finite to the run time;
+     one with the smallest spectral error, averaged over every check time
+     (or, with calib_objective = "eval_time", at the evaluation time only),
+     among those whose coarse runs all stay finite to the run time;
   7. evaluation: held-out seeds, energy spectra of coarse+closure, plain
      coarse and plain intermediate runs against the fine run, at every
      check time the runs stay finite.
@@ -202,18 +203,23 @@ def uniform_velocity(state, n, m=4):
     return g.reshape(3, n * m, n * m, n * m)  # [v][z][y][x]
 
 
-def energy_spectrum(vel, L):
-    """Shell-summed E(k), k = 1 .. N/2 in units of 2 pi / L (integer shells)."""
+def energy_spectrum(vel, L, backend="auto"):
+    """Shell-summed E(k), k = 1 .. N/2 in units of 2 pi / L (integer shells).
+    backend "gpu": the FFT on the GPU (torch.fft, hipFFT) in f32; "numpy":
+    numpy's f64 FFT on the host; "auto": the GPU when one is present."""
     N = vel.shape[1]
-    try:
+    if backend == "auto":
+        try:
+            import torch
+            backend = "gpu" if torch.cuda.is_available() else "numpy"
+        except ImportError:
+            backend = "numpy"
+    if backend == "gpu":
         import torch
-        if torch.cuda.is_available():
-            t = torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float32)).cuda()
-            uh = torch.fft.rfftn(t, dim=(1, 2, 3)) / (N ** 3)
-            e = (uh.abs() ** 2).sum(0).cpu().numpy().astype(np.float64)
-        else:
-            raise ImportError
-    except ImportError:
+        t = torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float32)).cuda()
+        uh = torch.fft.rfftn(t, dim=(1, 2, 3)) / (N ** 3)
+        e = (uh.abs() ** 2).sum(0).cpu().numpy().astype(np.float64)
+    else:
         uh = np.fft.rfftn(vel, axes=(1, 2, 3)) / (N ** 3)
         e = (np.abs(uh) ** 2).sum(0)
     k1 = np.fft.fftfreq(N, 1.0 / N)
@@ -244,7 +250,8 @@ def spectral_error(E, E_ref, k_max):
 DEFAULT = dict(L=TWO_PI, mach=0.1, re=400.0, n_modes=64, amplitude=0.1, kmax=4, flux="slau2", fine=48, ratio=3,
                mid=32, snap_times=[5.0, 10.0, 15.0, 20.0, 25.0, 30.0, 35.0, 40.0], eval_time=5.0, run_time=40.0,
                train_seeds=list(range(8)), eval_seeds=[100, 101], epochs=300, batch=65536, lr=2e-3,
-               update_interval=5, budget_kappa=1.0, calib_seeds=[0, 1], multipliers=[1, 2, 4, 8, 16, 32])
+               update_interval=5, budget_kappa=1.0, calib_seeds=[0, 1], multipliers=[1, 2, 4, 8, 16, 32],
+               calib_objective="all_times")
 
 
 def viscosity(cfg):
@@ -348,19 +355,22 @@ def calibrate(cfg, mlp, base_scale, log=print):
     te = times.index(cfg["eval_time"])
     refs = {seed: fine_spectra(cfg, seed, times)[0] for seed in cfg["calib_seeds"]}
     table = []
+    key = "err_all_times" if cfg.get("calib_objective", "eval_time") == "all_times" else "err_eval_time"
     for m in cfg["multipliers"]:
         mlp.set_output_scale(base_scale * m)
         rows = [coarse_errors(cfg, nc, seed, times, refs[seed], k_max, mlp) for seed in cfg["calib_seeds"]]
         stable = all(r["finite_until"] >= cfg["run_time"] for r in rows)
         e = [r["errs"][te] for r in rows]
         err = float(np.mean(e)) if all(x is not None for x in e) else None
-        table.append(dict(multiplier=m, err_eval_time=err, stable=stable,
+        ea = [x for r in rows for x in r["errs"]]
+        err_all = float(np.mean(ea)) if all(x is not None for x in ea) else None
+        table.append(dict(multiplier=m, err_eval_time=err, err_all_times=err_all, stable=stable,
                           finite_until=[r["finite_until"] for r in rows]))
-        log(f"calibration x{m}: mean spectral error at t={cfg['eval_time']} {err}, finite until "
-            f"{[r['finite_until'] for r in rows]}")
-    ok = [t for t in table if t["stable"] and t["err_eval_time"] is not None]
+        log(f"calibration x{m}: mean spectral error at t={cfg['eval_time']} {err}, over all check times "
+            f"{err_all}, finite until {[r['finite_until'] for r in rows]}")
+    ok = [t for t in table if t["stable"] and t[key] is not None]
     if ok:
-        best = min(ok, key=lambda t: t["err_eval_time"])
+        best = min(ok, key=lambda t: t[key])
     else:  # none stays finite: the most stable, then the most accurate
         best = max(table, key=lambda t: (min(t["finite_until"]), -(t["err_eval_time"] or 1e30)))
     mlp.set_output_scale(base_scale * best["multiplier"])
@@ -404,12 +414,16 @@ def main(argv=None):
     ap.add_argument("--dataset", default=None, help="skip the fine runs: load features/targets (.npz)")
     ap.add_argument("--flux", default=DEFAULT["flux"])
     ap.add_argument("--kmax", type=int, default=DEFAULT["kmax"])
+    ap.add_argument("--multipliers", type=float, nargs="+", default=DEFAULT["multipliers"],
+                    help="candidate output-scale multipliers of the a-posteriori calibration")
+    ap.add_argument("--calib-objective", choices=["all_times", "eval_time"], default=DEFAULT["calib_objective"],
+                    help="calibrate on the mean spectral error over every check time, or at the evaluation time")
     ap.add_argument("--multiplier", type=float, default=None,
                     help="output-scale multiplier (default: calibrated a posteriori on the calibration seeds)")
     args = ap.parse_args(argv)
     cfg = dict(DEFAULT, fine=args.fine, ratio=args.ratio, mid=args.mid, train_seeds=list(range(args.seeds)),
                epochs=args.epochs, snap_times=args.snap_times, eval_time=args.eval_time, flux=args.flux,
-               kmax=args.kmax)
+               kmax=args.kmax, multipliers=list(args.multipliers), calib_objective=args.calib_objective)
     os.makedirs(args.out, exist_ok=True)
     log = lambda s: print(s, flush=True)
     report = {"config": cfg}

@@ -267,3 +267,27 @@ def test_committed_closure_model_runs_on_the_coarse_mesh():
     assert st["out_of_range"] > 0
     # the eddy viscosity is a fraction of the molecular one at this Reynolds number
     assert 0.0 < np.median(mu[mu > 0]) < C.viscosity(cfg)
+
+
+def test_energy_spectrum_gpu_fft_matches_numpy():
+    """The spectra of the closure evaluation come from the GPU FFT (torch.fft,
+    hipFFT, f32): on a small multi-mode field they agree with numpy's f64 FFT
+    shell by shell, and both satisfy Parseval."""
+    from opendynamo_amd import closure as C
+    announce(test="energy_spectrum", n=6)
+    n, L = 6, TWO_PI
+    u = box_state(n=n, amp=0.1, seed=11, dtype=np.float64)
+    vel = C.uniform_velocity(u, n)  # 24^3 uniform grid
+    E_gpu = C.energy_spectrum(vel, L, backend="gpu")
+    E_np = C.energy_spectrum(vel, L, backend="numpy")
+    assert E_gpu.shape == E_np.shape == (12,)
+    # f32 FFT vs f64: a shell's relative error grows as sqrt(E_max / E_k) x f32 rounding,
+    # so shells above 1e-4 E_max are held to 1e-4 relative and every shell to 1e-6 E_max
+    big = E_np > 1e-4 * E_np.max()
+    assert big.sum() >= 5
+    assert np.all(np.abs(E_gpu[big] - E_np[big]) <= 1e-4 * E_np[big])
+    assert np.max(np.abs(E_gpu - E_np)) <= 1e-6 * E_np.max()
+    # Parseval over all shells including k = 0 and the corners beyond N/2 is
+    # checked on the host path: sum of shells <= 1/2 <|u|^2>
+    half_mean_sq = 0.5 * np.mean(np.sum(vel ** 2, axis=0))
+    assert E_np.sum() <= half_mean_sq * (1 + 1e-12)
