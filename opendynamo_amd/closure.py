@@ -2,7 +2,7 @@
 the spectral evaluation against the fine run (no reference counterpart).
 
 Pipeline (every step on the GPU through the library except the filter and
-the spectra, which are numpy/torch post-processing):
+the spectra, which are numpy post-processing):
 
   1. fine runs: plain FR P=3 NS on n_f^3 periodic boxes from a divergence-
      free Fourier field, snapshots at fixed times;
@@ -203,17 +203,13 @@ def uniform_velocity(state, n, m=4):
     return g.reshape(3, n * m, n * m, n * m)  # [v][z][y][x]
 
 
-def energy_spectrum(vel, L, backend="auto"):
+def energy_spectrum(vel, L, backend="numpy"):
     """Shell-summed E(k), k = 1 .. N/2 in units of 2 pi / L (integer shells).
-    backend "gpu": the FFT on the GPU (torch.fft, hipFFT) in f32; "numpy":
-    numpy's f64 FFT on the host; "auto": the GPU when one is present."""
+    backend "numpy": numpy's f64 FFT on the host (what the pipeline uses: its
+    grids are at most 192^3); "gpu": torch.fft (hipFFT) in f32, usable only
+    in a process where torch opens the GPU before this library does (torch
+    carries its own HIP runtime)."""
     N = vel.shape[1]
-    if backend == "auto":
-        try:
-            import torch
-            backend = "gpu" if torch.cuda.is_available() else "numpy"
-        except ImportError:
-            backend = "numpy"
     if backend == "gpu":
         import torch
         t = torch.from_numpy(np.ascontiguousarray(vel, dtype=np.float32)).cuda()

@@ -8,8 +8,11 @@
 //            4 zL, 5 zR              sequential from 0
 //            6 xL, 7 xR              sequential from the first product
 //           (face point fp: x j+4k, y i+4k, z i+4j)
-//   fv      [E][6][4][16]    CT   viscous flux components 1..4 at the
-//                                 element's own faces xL,xR,yL,yR,zL,zR
+//   fv      [E][640]         CT   per element: [6][4][16] viscous flux
+//                                 components 1..4 at the element's own faces
+//                                 xL,xR,yL,yR,zL,zR, then [4][64] the volume
+//                                 viscous-flux divergence of components 1..4
+//                                 at the solution points (fast f32 build only)
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -18,7 +21,8 @@ namespace od {
 
 constexpr int kTraceFaces = 8;
 constexpr int kTraceStride = kTraceFaces * 5 * 16;  // per element
-constexpr int kFvStride = 6 * 4 * 16;               // per element
+constexpr int kFvFaces = 6 * 4 * 16;                // face part, per element
+constexpr int kFvStride = kFvFaces + 4 * 64;        // + volume divergence part
 
 template <class C>
 struct Ops {

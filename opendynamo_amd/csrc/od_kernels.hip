@@ -7,8 +7,12 @@
 //                     at the element's 6 faces -> fv  (replaces the gradient
 //                     half of compute_rhs_volume_kernel and the viscous half of
 //                     the face kernels)
-//   od_update_kernel  one wave64 per element: gradient again (register only),
-//                     volume flux divergence, interface fluxes at all 6 own
+//                     In the fast f32 build it also forms the divergence of
+//                     the volume viscous fluxes (-> fv, after the face part)
+//   od_update_kernel  one wave64 per element: gradient again (register only;
+//                     not in the fast f32 build, which takes the volume
+//                     viscous divergence from the visc kernel), volume flux
+//                     divergence, interface fluxes at all 6 own
 //                     faces (SLAU2 / AUSM+ / Rusanov, slip wall), FR
 //                     correction, SSP-RK3 blend, face traces of the new state
 //                     (replaces the rest of the volume kernel, the x/y/wall/z
@@ -416,6 +420,14 @@ __device__ __forceinline__ C trace4_swz(const C* q, int base, int stride, C c0, 
 // sides of a face see the same two traces either way, so the interface
 // flux stays single-valued (conservative).
 constexpr bool kOneX = OD_FAST;
+// Fast f32 build: the visc kernel, which holds the corrected gradient anyway,
+// also forms the divergence of the volume viscous fluxes and stores it after
+// the element's face viscous fluxes (fv + kFvFaces); the update kernel then
+// takes the divergence of the Euler fluxes only and needs no gradient of its
+// own.  The strict build keeps the reference's single divergence of
+// F_e - F_v (rhs.cu:229-266) and its rounding.
+template <class CT>
+constexpr bool kSplitVisc = OD_FAST && std::is_same<CT, float>::value;
 // Order of the BR1 traces (rhs.cu:385-455): x, y pairwise; z sequential from 0.
 __device__ __forceinline__ constexpr int br1_order(int d) { return (d == 2 && !kOneX) ? 1 : 0; }
 // Order of the Riemann inputs and gradient slices (rhs.cu:126-180, 540-543,
@@ -829,6 +841,47 @@ __device__ __forceinline__ void write_traces_own(const CT (*sUe)[64], CT* dst, i
     }
 }
 
+// Divergence of the volume viscous fluxes at the lane's point (the F_v half
+// of accumulate_volume_rhs_shfl / _z, rhs.cu:229-266) from the corrected
+// gradient g, x then y then z, components 1..4 (F_v of the mass equation is
+// zero) -> the element's fv block after its face part.  S: this wave's
+// 256-float LDS staging area.  Used by the fast f32 build (kSplitVisc).
+template <class UT, class CT>
+__device__ __forceinline__ void visc_divergence(const StageArgs<UT, CT>& a, const Tile& tl, const CT (*sUe)[64],
+                                                const CT* g, CT* S, int wave, int i, int j, int k, int pt) {
+    CT st[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) st[v] = sUe[v][pt];
+    CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+    if (a.mu_t) mu = mu + a.mu_t[tile_eidx(tl, wave)];
+    const ViscPoint<CT> vp = viscous_point(st, g, mu, 7);
+    CT acc[4] = {(CT)0, (CT)0, (CT)0, (CT)0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        CT fvv[5];
+        viscous_flux_dir(d, vp, fvv);
+        if (d > 0) wave_sync();  // the previous direction's reads are done
+#pragma unroll
+        for (int q = 0; q < 4; ++q) S[q * 64 + pt] = fvv[1 + q];
+        wave_sync();
+        const int idx = d == 0 ? i : (d == 1 ? j : k);
+        const int base = d == 0 ? 4 * j + 16 * k : (d == 1 ? i + 16 * k : i + 4 * j);
+        const int stride = d == 0 ? 1 : (d == 1 ? 4 : 16);
+        const CT jac = d == 0 ? a.jx : (d == 1 ? a.jy : a.jz);
+        const CT* Dd = coef_lds<CT>().D[idx];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            CT dd = 0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) dd += Dd[m] * S[q * 64 + base + m * stride];
+            acc[q] += dd * jac;
+        }
+    }
+    CT* dst = a.fv + tile_eidx(tl, wave) * kFvStride + kFvFaces + pt;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q * 64] = acc[q];
+}
+
 // ---------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------
@@ -860,17 +913,18 @@ struct ViscLds {
 template <class CT, bool VISC>
 struct UpdateLds {
     static constexpr bool kPF = std::is_same<CT, float>::value;
+    static constexpr bool kJmp = VISC && !kSplitVisc<CT>;  // BR1 jumps formed in this kernel
     float pf[kPF ? kPfFloatsUpdate : 4];
     CT sU[kEpb][5][64];
-    CT sJmp[kEpb][6][5][16];
+    CT sJmp[kJmp ? kEpb : 0][6][5][16];
     CT sJ_own[kPF ? 0 : kEpb][6][5][16];  // zero-length when aliased
-    CT sF_own[VISC ? 0 : kEpb][5][64];
+    CT sF_own[kJmp ? 0 : kEpb][5][64];
     __device__ CT (*sJ())[6][5][16] {
         if constexpr (kPF) return reinterpret_cast<CT(*)[6][5][16]>(pf + kPfA);
         else return sJ_own;
     }
     __device__ CT (*sF(int w))[64] {
-        if constexpr (VISC) return reinterpret_cast<CT(*)[64]>(&sJmp[w][0][0][0]);
+        if constexpr (kJmp) return reinterpret_cast<CT(*)[64]>(&sJmp[w][0][0][0]);
         else return sF_own[w];
     }
 };
@@ -911,6 +965,9 @@ __device__ __forceinline__ void visc_body(const StageArgs<UT, CT>& a, const Tile
     __syncthreads();
     CT g[15];
     if (tile_valid(tl, wave)) lane_gradient(a, L.sU[wave], L.sJmp[wave], i, j, k, g);
+    if constexpr (kSplitVisc<CT>) {
+        if (tile_valid(tl, wave)) visc_divergence(a, tl, L.sU[wave], g, L.pf + wave * 256, wave, i, j, k, pt);
+    }
     if constexpr (PF) __syncthreads();  // sG overwrites the jumps the other wave may still read
     if (tile_valid(tl, wave)) {
 #pragma unroll
@@ -950,8 +1007,10 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
 
+    // with kSplitVisc the volume viscous terms come from the visc kernel
+    constexpr bool VOLV = VISC && !kSplitVisc<CT>;
     CT g[15];
-    if (VISC) {
+    if constexpr (VOLV) {
         run_br1_jumps<PF>(a, L.sU, L.sJmp, tl, pfc, wave, pt);
         __syncthreads();
         lane_gradient(a, L.sU[wave], L.sJmp[wave], i, j, k, g);
@@ -960,18 +1019,24 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 #pragma unroll
     for (int v = 0; v < 5; ++v) st[v] = L.sU[wave][v][pt];
     CT mu = (CT)0;
-    if (VISC) mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
-    if (VISC && a.mu_t) mu = mu + (mine ? a.mu_t[tile_eidx(tl, wave)] : (CT)0);
+    if (VOLV) mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+    if (VOLV && a.mu_t) mu = mu + (mine ? a.mu_t[tile_eidx(tl, wave)] : (CT)0);
+    CT vdiv[4] = {(CT)0, (CT)0, (CT)0, (CT)0};
+    if (VISC && !VOLV && mine) {
+        const CT* src = a.fv + tile_eidx(tl, wave) * kFvStride + kFvFaces + pt;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vdiv[q] = src[q * 64];
+    }
 
     // Volume flux divergence, x then y then z (rhs.cu:229-266, 490-510).
     CT rs[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
     ViscPoint<CT> vp;
-    if (VISC) vp = viscous_point(st, g, mu, 7);
+    if (VOLV) vp = viscous_point(st, g, mu, 7);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         CT fe[5], F[5];
         euler_flux(d, st, fe);
-        if (VISC) {
+        if (VOLV) {
             CT fvv[5];
             viscous_flux_dir(d, vp, fvv);
 #pragma unroll
@@ -1002,6 +1067,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         CT val = -rs[v];
+        if (VISC && !VOLV && v > 0) val += vdiv[v - 1];  // + div F_v from the visc kernel
         if (v == 1) val += a.bf;
         L.sF(wave)[v][pt] = a.round_rhs ? (CT)(float)val : val;  // RHS_REAL round trip (rhs.cu:509, rk.cu:52)
     }
@@ -1094,16 +1160,17 @@ __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
     visc_body(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
 }
 
-// Occupancy target of the update kernels.  f32 compute needs ~73-78 VGPRs
-// (6 waves per SIMD); the fast all-f32 single-box kernel and the batch
-// kernel's single-unit form are held to 72 (7 waves, a few registers spill;
-// measured ~1 % faster per step).  The strict build, f64 states and the
-// persistent batch kernel spill too much at 72 and keep their natural
-// allocation (the 4-wave cap does not bind); f64 compute needs ~145 VGPRs.
+// Occupancy target of the update kernels.  The strict f32 build needs ~78
+// VGPRs (6 waves per SIMD).  The fast all-f32 single-box kernel and the batch
+// kernel's single-unit form, which need no gradient there (kSplitVisc), are
+// held to 64 (8 waves; 2 registers spill; their LDS, 9.1 KB per work-group,
+// allows 8).  The strict build, f64 states and the persistent batch kernel
+// keep their natural allocation (the 4-wave cap does not bind); f64 compute
+// needs ~145 VGPRs.
 template <class CT>
 constexpr int kMinWaves = sizeof(CT) == 4 ? 4 : 1;
 template <class UT, class CT>
-constexpr int kMinWavesUpdate = (OD_FAST && sizeof(UT) == 4 && sizeof(CT) == 4) ? 7 : kMinWaves<CT>;
+constexpr int kMinWavesUpdate = (OD_FAST && sizeof(UT) == 4 && sizeof(CT) == 4) ? 8 : kMinWaves<CT>;
 
 template <class UT, class CT, bool VISC>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWavesUpdate<UT, CT>))) void od_update_kernel(
