@@ -890,7 +890,11 @@ __device__ __forceinline__ void visc_divergence(const StageArgs<UT, CT>& a, cons
 // With the prefetch (f32 compute) the gradient tile sG reuses the prefetched
 // traces and the BR1 jumps (960 floats each, contiguous): element 0's
 // gradients take the prefetch buffer, element 1's the jumps, written only
-// after both waves have finished reading them (a barrier).
+// after both waves have finished reading them (a barrier).  Between the
+// jumps and the gradient tile the fast f32 build stages the volume viscous
+// fluxes of element w in pf[256 w, 256 w + 256) (visc_divergence): the
+// prefetched traces are consumed by then and sG is written only after the
+// next barrier.
 template <class CT>
 struct ViscLds {
     static constexpr bool kPF = std::is_same<CT, float>::value;
