@@ -89,7 +89,8 @@ od_status od_get_state_async(od_ctx* ctx, uint64_t* ticket);
 od_status od_wait(od_ctx* ctx, uint64_t ticket, const void** host_state);
 
 /* Enqueue n_steps RK3 steps of size dt (the f32 config value widened).
- * Asynchronous; one hipGraph launch per step. */
+ * Asynchronous; replays of a captured 8-step hipGraph plus a 1-step one
+ * (one 1-step replay per step while a closure network is attached). */
 od_status od_step(od_ctx* ctx, double dt, uint32_t n_steps);
 od_status od_synchronize(od_ctx* ctx);
 

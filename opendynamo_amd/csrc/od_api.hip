@@ -173,26 +173,50 @@ od_status enqueue_step(od_ctx* c, double dt) {
     return OD_OK;
 }
 
-od_status ensure_graph(od_ctx* c, double dt) {
-    if (c->graph && c->graph_dt == dt && c->graph_mu_t == c->mu_t_on) return OD_OK;
-    if (c->graph) {
-        HIP_TRY(hipGraphExecDestroy(c->graph));
-        c->graph = nullptr;
-    }
+// Steps per replay of the multi-step graph: consecutive graph launches leave
+// ~10 us of idle GPU between them, the kernels inside one graph none, and
+// every step has the same buffer roles, so n steps run as n / 8 replays of an
+// 8-step graph plus n % 8 of the one-step graph (same kernels, same order).
+constexpr uint32_t kStepsPerGraph = 8;
+
+od_status capture_steps(od_ctx* c, double dt, uint32_t steps, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    od_status s = enqueue_step(c, dt);
+    od_status s = OD_OK;
+    for (uint32_t q = 0; q < steps && s == OD_OK; ++q) s = enqueue_step(c, dt);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (s != OD_OK) {
         if (g) (void)hipGraphDestroy(g);
         return s;
     }
     if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return OD_OK;
+}
+
+od_status ensure_graph(od_ctx* c, double dt) {
+    if (c->graph && c->graph_k && c->graph_dt == dt && c->graph_mu_t == c->mu_t_on) return OD_OK;
+    for (hipGraphExec_t* gx : {&c->graph, &c->graph_k}) {
+        if (*gx) {
+            HIP_TRY(hipGraphExecDestroy(*gx));
+            *gx = nullptr;
+        }
+    }
+    od_status s = capture_steps(c, dt, 1, &c->graph);
+    if (s == OD_OK) s = capture_steps(c, dt, kStepsPerGraph, &c->graph_k);
+    if (s != OD_OK) return s;
     c->graph_dt = dt;
     c->graph_mu_t = c->mu_t_on;
+    return OD_OK;
+}
+
+// Enqueue n steps of the captured graphs on the library stream.
+od_status launch_steps(od_ctx* c, uint32_t n) {
+    uint32_t q = 0;
+    for (; q + kStepsPerGraph <= n; q += kStepsPerGraph) HIP_TRY(hipGraphLaunch(c->graph_k, c->stream));
+    for (; q < n; ++q) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
     return OD_OK;
 }
 
@@ -290,6 +314,7 @@ void od_destroy(od_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->io_stream) (void)hipStreamSynchronize(c->io_stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->graph_k) (void)hipGraphExecDestroy(c->graph_k);
     for (void* q : c->u)
         if (q) (void)hipFree(q);
     for (void* q : c->tr)
@@ -388,8 +413,13 @@ od_status od_step(od_ctx* c, double dt, uint32_t n_steps) {
     HIP_TRY(hipSetDevice(c->device));
     od_status s = ensure_graph(c, dt);
     if (s != OD_OK) return s;
-    for (uint32_t q = 0; q < n_steps; ++q) {
-        if (c->mlp && c->steps_done % (uint64_t)c->closure.update_interval == 0) {
+    if (!c->mlp) {
+        s = launch_steps(c, n_steps);
+        if (s == OD_OK) c->steps_done += n_steps;
+        return s;
+    }
+    for (uint32_t q = 0; q < n_steps; ++q) {  // the closure refreshes mu_t between steps
+        if (c->steps_done % (uint64_t)c->closure.update_interval == 0) {
             s = od_closure_update_internal(c, dt * c->closure.update_interval);
             if (s != OD_OK) return s;
         }
@@ -462,7 +492,12 @@ od_status od_time_steps(od_ctx* c, double dt, uint32_t n_steps, double* ms_total
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
     HIP_TRY(hipEventRecord(a, c->stream));
-    for (uint32_t q = 0; q < n_steps; ++q) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    s = launch_steps(c, n_steps);
+    if (s != OD_OK) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return s;
+    }
     HIP_TRY(hipEventRecord(b, c->stream));
     HIP_TRY(hipEventSynchronize(b));
     float ms = 0.f;

@@ -19,7 +19,8 @@ struct od_ctx {
     double* sums_out = nullptr;
     od::host::HostOps ops;
     bool state_set = false;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;    // one RK3 step
+    hipGraphExec_t graph_k = nullptr;  // kStepsPerGraph RK3 steps
     double graph_dt = 0.0;
     void* pinned[3] = {nullptr, nullptr, nullptr};
     hipEvent_t slot_done[3] = {nullptr, nullptr, nullptr};

@@ -231,3 +231,26 @@ def test_fast_math_deterministic_and_resume():
         s.step(1e-3, 10)
         resumed = s.get_state()
     assert resumed.tobytes() == full.tobytes()
+
+
+@pytest.mark.parametrize("real,fast", [(4, 1), (8, 0)])
+def test_multi_step_graph_equals_single_steps(real, fast):
+    """od_step runs n steps as replays of an 8-step graph plus one-step
+    replays; the state is byte-identical to n separate one-step calls and
+    to any other split of the same n."""
+    nx, ny, nz = 4, 3, 5
+    lx, ly, lz = TWO_PI, 2.0, np.pi
+    announce(test="multi_step_graph", real=real, fast=fast)
+    pg = od.params(nx, ny, nz, lx, ly, lz, u_real=real, flux="slau2", y_wall=True, viscosity_model=1, mu_ref=1e-3,
+                   fast_math=fast)
+    u = smooth_state(nx, ny, nz, lx, ly, lz, seed=4, amp=0.05, dtype=np.float32 if real == 4 else np.float64)
+    out = []
+    for split in ([17], [1] * 17, [8, 9], [3, 8, 6]):
+        with od.Solver(pg) as s:
+            s.set_state(u)
+            for n in split:
+                s.step(1e-3, n)
+            out.append(s.get_state())
+    assert np.isfinite(out[0]).all()
+    for o in out[1:]:
+        assert o.tobytes() == out[0].tobytes()
