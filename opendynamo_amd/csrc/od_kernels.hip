@@ -564,12 +564,17 @@ __device__ __forceinline__ int64_t tile_nb(const Tile& t, int le, int side) {
 
 // Neighbour across face f (0..5) of local element le, -1 = wall; selects on
 // named fields only.
-__device__ __forceinline__ int elem_nb(const Elem& a, int f) {
-    return f == 0 ? a.xl : f == 1 ? a.xr : f == 2 ? a.yl : f == 3 ? a.yr : f == 4 ? a.zl : a.zr;
-}
+// Neighbour across face f (lane-varying) of tile element le (lane-varying):
+// selects among the twelve wave-uniform indices, taken by value so that the
+// selects stay in registers (a nested conditional on the struct fields
+// compiles to divergent branches, a select of field references to a
+// scratch-memory load).
 __device__ __forceinline__ int64_t tile_nb_rt(const Tile& t, int le, int f) {
-    const int n0 = elem_nb(t.el0, f), n1 = elem_nb(t.el1, f);
-    return (int64_t)(le ? n1 : n0);
+    const int xl = le ? t.el1.xl + 0 : t.el0.xl + 0, xr = le ? t.el1.xr + 0 : t.el0.xr + 0;
+    const int yl = le ? t.el1.yl + 0 : t.el0.yl + 0, yr = le ? t.el1.yr + 0 : t.el0.yr + 0;
+    const int zl = le ? t.el1.zl + 0 : t.el0.zl + 0, zr = le ? t.el1.zr + 0 : t.el0.zr + 0;
+    const int x = (f & 1) ? xr : xl, y = (f & 1) ? yr : yl, z = (f & 1) ? zr : zl;
+    return (int64_t)(f < 2 ? x : (f < 4 ? y : z));
 }
 
 // ---------------------------------------------------------------------------
@@ -616,6 +621,7 @@ template <class UT, class CT>
 __device__ __forceinline__ void prefetch_issue(const StageArgs<UT, CT>& a, const Tile& tl, float* pf, int n_chunks,
                                                int wave, int lane) {
     const int n_inst = (n_chunks + 63) / 64;
+    wave = __builtin_amdgcn_readfirstlane(wave);  // wave-uniform: a scalar loop
     const int first = wave == 0 ? 0 : (n_inst + 1) / 2;
     const int last = wave == 0 ? (n_inst + 1) / 2 : n_inst;
     for (int q = first; q < last; ++q) {
