@@ -818,6 +818,12 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     }
 }
 
+// Index of this wave in its work-group, as a wave-uniform (scalar) value: the
+// compiler cannot tell that threadIdx.x >> 6 is the same in every lane, and
+// per-wave conditions derived from it (the wave's element is valid, the
+// wave's face directions) would otherwise branch on the execution mask.
+__device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // Orders this wave's earlier LDS writes before its later LDS reads (of
 // other lanes' data): a wave's LDS operations execute in order, so this only
 // has to keep the compiler from moving them.
@@ -1161,13 +1167,13 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 template <class UT, class CT>
 __global__ __launch_bounds__(128) void od_trace_kernel(StageArgs<UT, CT> a) {
     __shared__ CT sU[kEpb][5][64];
-    trace_body(a, make_tile(a), sU, threadIdx.x >> 6, threadIdx.x & 63);
+    trace_body(a, make_tile(a), sU, wave_index(), threadIdx.x & 63);
 }
 
 template <class UT, class CT>
 __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
     __shared__ ViscLds<CT> L;
-    visc_body(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
+    visc_body(a, make_tile(a), L, wave_index(), threadIdx.x & 63);
 }
 
 // Occupancy target of the update kernels.  The strict f32 build needs ~78
@@ -1186,7 +1192,7 @@ template <class UT, class CT, bool VISC>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWavesUpdate<UT, CT>))) void od_update_kernel(
     StageArgs<UT, CT> a) {
     __shared__ UpdateLds<CT, VISC> L;
-    update_body<UT, CT, VISC>(a, make_tile(a), L, threadIdx.x >> 6, threadIdx.x & 63);
+    update_body<UT, CT, VISC>(a, make_tile(a), L, wave_index(), threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
@@ -1294,7 +1300,7 @@ template <class UT, class CT, int PAIRS, bool LOOP>
 __global__ __launch_bounds__(128 * PAIRS) void od_batch_visc_kernel(StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ ViscLds<CT> L[PAIRS];
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
-    const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
+    const int wave = wave_index() & 1, pt = threadIdx.x & 63;
     if constexpr (!LOOP) {
         StageArgs<UT, CT> ab;
         Tile tl;
@@ -1323,7 +1329,7 @@ __global__ __launch_bounds__(128 * PAIRS) __attribute__((amdgpu_waves_per_eu(LOO
     StageArgs<UT, CT> a, BatchArgs<UT, CT> b) {
     __shared__ UpdateLds<CT, VISC> L[PAIRS];
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
-    const int wave = (threadIdx.x >> 6) & 1, pt = threadIdx.x & 63;
+    const int wave = wave_index() & 1, pt = threadIdx.x & 63;
     if constexpr (!LOOP) {
         StageArgs<UT, CT> ab;
         Tile tl;
