@@ -1179,8 +1179,8 @@ __global__ __launch_bounds__(128) void od_visc_kernel(StageArgs<UT, CT> a) {
 // Occupancy target of the update kernels.  The strict f32 build needs ~78
 // VGPRs (6 waves per SIMD).  The fast all-f32 single-box kernel and the batch
 // kernel's single-unit form, which need no gradient there (kSplitVisc), are
-// held to 64 (8 waves; 2 registers spill; their LDS, 9.1 KB per work-group,
-// allows 8).  The strict build, f64 states and the persistent batch kernel
+// held to 64 (8 waves; they need 61 with no spills; their LDS, 9.1 KB per
+// work-group, allows 8).  The strict build, f64 states and the persistent batch kernel
 // keep their natural allocation (the 4-wave cap does not bind); f64 compute
 // needs ~145 VGPRs.
 template <class CT>
