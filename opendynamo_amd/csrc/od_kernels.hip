@@ -428,6 +428,27 @@ constexpr bool kOneX = OD_FAST;
 // F_e - F_v (rhs.cu:229-266) and its rounding.
 template <class CT>
 constexpr bool kSplitVisc = OD_FAST && std::is_same<CT, float>::value;
+// The x line through point (i, j, k) is the four consecutive lanes of the
+// point's quad, so the fast f32 build contracts it from registers with DPP
+// quad broadcasts instead of an LDS round trip: the same products summed in
+// the same order, so the results are those of the LDS form bit for bit.
+template <class CT>
+constexpr bool kDppX = OD_FAST && std::is_same<CT, float>::value;
+template <class CT>
+__device__ __forceinline__ CT quad_line(const CT* Dd, CT f) {
+    static_assert(std::is_same<CT, float>::value, "DPP lanes are 32-bit");
+    const int x = __float_as_int(f);
+    const CT b0 = __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x00, 0xF, 0xF, false));  // quad_perm [0,0,0,0]
+    const CT b1 = __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x55, 0xF, 0xF, false));  // [1,1,1,1]
+    const CT b2 = __int_as_float(__builtin_amdgcn_mov_dpp(x, 0xAA, 0xF, 0xF, false));  // [2,2,2,2]
+    const CT b3 = __int_as_float(__builtin_amdgcn_mov_dpp(x, 0xFF, 0xF, 0xF, false));  // [3,3,3,3]
+    CT dd = 0;
+    dd += Dd[0] * b0;
+    dd += Dd[1] * b1;
+    dd += Dd[2] * b2;
+    dd += Dd[3] * b3;
+    return dd;
+}
 // Order of the BR1 traces (rhs.cu:385-455): x, y pairwise; z sequential from 0.
 __device__ __forceinline__ constexpr int br1_order(int d) { return (d == 2 && !kOneX) ? 1 : 0; }
 // Order of the Riemann inputs and gradient slices (rhs.cu:126-180, 540-543,
@@ -872,7 +893,15 @@ __device__ __forceinline__ void visc_divergence(const StageArgs<UT, CT>& a, cons
     for (int d = 0; d < 3; ++d) {
         CT fvv[5];
         viscous_flux_dir(d, vp, fvv);
-        if (d > 0) wave_sync();  // the previous direction's reads are done
+        if constexpr (kDppX<CT>) {
+            if (d == 0) {
+                const CT* Dx = coef_lds<CT>().D[i];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] += quad_line(Dx, fvv[1 + q]) * a.jx;
+                continue;
+            }
+        }
+        if (d > (kDppX<CT> ? 1 : 0)) wave_sync();  // the previous direction's reads are done
 #pragma unroll
         for (int q = 0; q < 4; ++q) S[q * 64 + pt] = fvv[1 + q];
         wave_sync();
@@ -1061,7 +1090,15 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 #pragma unroll
             for (int v = 0; v < 5; ++v) F[v] = fe[v];
         }
-        if (d > 0) __syncthreads();
+        if constexpr (kDppX<CT>) {
+            if (d == 0) {
+                const CT* Dx = coef_lds<CT>().D[i];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) rs[v] += quad_line(Dx, F[v]) * a.jx;
+                continue;
+            }
+        }
+        if (d > (kDppX<CT> ? 1 : 0)) __syncthreads();
 #pragma unroll
         for (int v = 0; v < 5; ++v) L.sF(wave)[v][pt] = F[v];
         __syncthreads();
