@@ -8,11 +8,12 @@
 //            4 zL, 5 zR              sequential from 0
 //            6 xL, 7 xR              sequential from the first product
 //           (face point fp: x j+4k, y i+4k, z i+4j)
-//   fv      [E][640]         CT   per element: [6][4][16] viscous flux
-//                                 components 1..4 at the element's own faces
-//                                 xL,xR,yL,yR,zL,zR, then [4][64] the volume
-//                                 viscous-flux divergence of components 1..4
-//                                 at the solution points (fast f32 build only)
+//   fv      [E][640]         CT   per element: [6][16][4] viscous flux
+//                                 components 1..4 at the face points of the
+//                                 element's own faces xL,xR,yL,yR,zL,zR, then
+//                                 [64][4] the volume viscous-flux divergence of
+//                                 components 1..4 at the solution points (fast
+//                                 f32 build only); component fastest
 #pragma once
 #include <cstddef>
 #include <cstdint>

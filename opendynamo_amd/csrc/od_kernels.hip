@@ -748,6 +748,14 @@ __device__ __forceinline__ void run_br1_jumps(const StageArgs<UT, CT>& a, const 
     }
 }
 
+// The four viscous flux components 1..4 of one point, stored together
+// (fv is point-major, component fastest) so that each lane moves them with
+// one 16-byte (f32) access instead of four 4-byte ones.
+template <class C>
+struct alignas(4 * sizeof(C)) FvQuad {
+    C c[4];
+};
+
 // Viscous normal flux at the element faces of direction D (the
 // dispatch_viscous_flux calls of compute_face_j / compute_face_j_wall,
 // rhs.cu:284-285, 318) from the Riemann-order state trace and the gradient
@@ -768,9 +776,8 @@ __device__ __forceinline__ void fv_pass(const StageArgs<UT, CT>& a, const CT (*s
     CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
     if (a.mu_t) mu = mu + a.mu_t[tile_eidx(tl, t.le)];
     viscous_flux(D, st, gt, mu, fvv);
-    CT* dst = a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) dst[q * 16] = fvv[1 + q];
+    FvQuad<CT>* dst = reinterpret_cast<FvQuad<CT>*>(a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp * 4);
+    *dst = FvQuad<CT>{{fvv[1], fvv[2], fvv[3], fvv[4]}};
 }
 
 // Interface flux at the element faces of direction D -> own-side correction
@@ -804,10 +811,14 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
     }
     CT fvo[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0}, fvn[5] = {(CT)0, (CT)0, (CT)0, (CT)0, (CT)0};
     if (VISC) {
+        const FvQuad<CT> o =
+            *reinterpret_cast<const FvQuad<CT>*>(a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp * 4);
+        FvQuad<CT> n = o;
+        if (!wall) n = *reinterpret_cast<const FvQuad<CT>*>(a.fv + nb * kFvStride + (t.f ^ 1) * 64 + t.fp * 4);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            fvo[1 + q] = a.fv[tile_eidx(tl, t.le) * kFvStride + t.f * 64 + q * 16 + t.fp];
-            fvn[1 + q] = wall ? fvo[1 + q] : a.fv[nb * kFvStride + (t.f ^ 1) * 64 + q * 16 + t.fp];
+            fvo[1 + q] = o.c[q];
+            fvn[1 + q] = n.c[q];
         }
     }
     CT L[5], R[5];
@@ -918,9 +929,8 @@ __device__ __forceinline__ void visc_divergence(const StageArgs<UT, CT>& a, cons
             acc[q] += dd * jac;
         }
     }
-    CT* dst = a.fv + tile_eidx(tl, wave) * kFvStride + kFvFaces + pt;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) dst[q * 64] = acc[q];
+    FvQuad<CT>* dst = reinterpret_cast<FvQuad<CT>*>(a.fv + tile_eidx(tl, wave) * kFvStride + kFvFaces + pt * 4);
+    *dst = FvQuad<CT>{{acc[0], acc[1], acc[2], acc[3]}};
 }
 
 // ---------------------------------------------------------------------------
@@ -1068,9 +1078,10 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
     if (VOLV && a.mu_t) mu = mu + (mine ? a.mu_t[tile_eidx(tl, wave)] : (CT)0);
     CT vdiv[4] = {(CT)0, (CT)0, (CT)0, (CT)0};
     if (VISC && !VOLV && mine) {
-        const CT* src = a.fv + tile_eidx(tl, wave) * kFvStride + kFvFaces + pt;
+        const FvQuad<CT> src =
+            *reinterpret_cast<const FvQuad<CT>*>(a.fv + tile_eidx(tl, wave) * kFvStride + kFvFaces + pt * 4);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) vdiv[q] = src[q * 64];
+        for (int q = 0; q < 4; ++q) vdiv[q] = src.c[q];
     }
 
     // Volume flux divergence, x then y then z (rhs.cu:229-266, 490-510).
