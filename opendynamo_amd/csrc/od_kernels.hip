@@ -885,6 +885,24 @@ __device__ __forceinline__ void write_traces_own(const CT (*sUe)[64], CT* dst, i
     }
 }
 
+// State tile of element e into LDS, sUe[v][pt] = u[v][e][pt].  With an f32
+// state and f32 compute each wave copies its element with LDS-DMA
+// (global_load_lds_dword: no registers; the data lands with the
+// neighbour-trace prefetch and is covered by the same wait); otherwise
+// through registers with the cast to the compute type.
+template <class UT, class CT>
+__device__ __forceinline__ void load_state_tile(const UT* u, int64_t NP, int64_t e, CT (*sUe)[64], int pt) {
+    if constexpr (std::is_same<UT, float>::value && std::is_same<CT, float>::value) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(u + v * NP + e * 64 + pt),
+                                             (__attribute__((address_space(3))) void*)&sUe[v][0], 4, 0, 0);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sUe[v][pt] = (CT)u[v * NP + e * 64 + pt];
+    }
+}
+
 // Divergence of the volume viscous fluxes at the lane's point (the F_v half
 // of accumulate_volume_rhs_shfl / _z, rhs.cu:229-266) from the corrected
 // gradient g, x then y then z, components 1..4 (F_v of the mass equation is
@@ -1009,10 +1027,7 @@ __device__ __forceinline__ void visc_body(const StageArgs<UT, CT>& a, const Tile
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const CT cf = coef_load<CT>((int)threadIdx.x);
     if constexpr (PF) prefetch_issue(a, tl, L.pf, kPfChunksVisc, wave, pt);
-    if (tile_valid(tl, wave)) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)a.uin[v * NP + tile_eidx(tl, wave) * 64 + pt];
-    }
+    if (tile_valid(tl, wave)) load_state_tile(a.uin, NP, tile_eidx(tl, wave), L.sU[wave], pt);
     coef_store<CT>((int)threadIdx.x, cf);
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
@@ -1053,10 +1068,19 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 
     constexpr bool SAME = std::is_same<UT, CT>::value;  // sU holds the state exactly
     UT ucur[5];
+    if constexpr (SAME && PF) {  // f32 state and compute: LDS-DMA, read back from sU below
+        if (mine) {
+            load_state_tile(a.uin, NP, tile_eidx(tl, wave), L.sU[wave], pt);
+        } else {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        ucur[v] = mine ? a.uin[v * NP + gbase] : (UT)1;
-        L.sU[wave][v][pt] = (CT)ucur[v];
+            for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)1;
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            ucur[v] = mine ? a.uin[v * NP + gbase] : (UT)1;
+            L.sU[wave][v][pt] = (CT)ucur[v];
+        }
     }
     coef_store<CT>((int)threadIdx.x, cf);
     if constexpr (PF) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
