@@ -123,12 +123,37 @@ def hip_copy_calibration(nbytes):
             "launches": 3}
 
 
-def cpu_threads():
+def host_cpu():
+    """The host's CPUs as this process sees them: affinity count, logical CPU
+    count, the CPU model string and the thread allotment of the environment
+    (OMP_NUM_THREADS; the GPU hosts give each GPU's jobs a share of 16)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        affinity = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    env = os.environ.get("OMP_NUM_THREADS")
+    allot = int(env) if env and env.isdigit() and int(env) > 0 else None
+    return {"affinity_cpus": affinity, "logical_cpus": os.cpu_count(), "model": model,
+            "omp_num_threads": allot}
+
+
+def cpu_threads():
+    """Oracle worker threads: every CPU this process may run on, limited to the
+    environment's allotment (OMP_NUM_THREADS) where one is set."""
+    h = host_cpu()
+    n = h["affinity_cpus"]
+    if h["omp_num_threads"]:
+        n = min(n, h["omp_num_threads"])
+    return max(1, n)
 
 
 def run_ensemble(args):
@@ -285,6 +310,12 @@ def main():
     roofline = {
         "bound": "hbm", "kernel": "od_update_kernel", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+        # the roofline is priced on HBM bytes; what limits the kernels as
+        # measured is VALU issue and HBM together (rocprofv3 SQ_INSTS_VALU
+        # and FETCH/WRITE_SIZE in the committed profile)
+        "limiter": "valu+hbm (measured)",
+        "traffic_source": "profiles/fr_step_kernels_48cubed_f32_pmc.txt: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                          "of od_update_kernel in separate profiling runs, copy-calibrated (not this run)",
         "algorithmic_bytes_per_launch": alg_bytes_launch, "launch_ms": round(upd_ms, 5),
         # the profiled HBM bytes per launch over the live launch time: how much
         # of the peak the kernel actually moves (the design's face traces and
@@ -326,10 +357,16 @@ def main():
         ref = O.step(po, u64, dt, args.cpu_steps)
         cpu_s = time.perf_counter() - t0
         cpu_value = dof * args.cpu_steps / cpu_s / 1e9
+        hc = host_cpu()
         out["cpu_baseline"] = {
             "value": round(cpu_value, 6), "unit": "GDOF·steps/s", "cores": threads, "kind": "port",
             "sample": f"{args.cpu_steps} RK3 steps of the same 48^3 workload from the same IC, FP64 op-for-op "
-                      f"oracle, {threads} threads, {cpu_s:.1f} s",
+                      f"oracle (C++ std::thread over element ranges, -O3 -march=native, no FP contraction), "
+                      f"{threads} threads, {cpu_s:.1f} s",
+            "per_thread_value": round(cpu_value / threads, 7),
+            "host": hc,
+            "threads_note": "threads = the CPUs this process may use, capped by OMP_NUM_THREADS when the "
+                            "environment sets it (the GPU host allots 16 CPUs to each GPU's jobs)",
         }
         out["speedup_vs_cpu"] = round(value / cpu_value, 1)
         # per-variable error of the f32 GPU pipelines against the FP64 oracle

@@ -15,6 +15,7 @@
  *   od_get_state         solver.rs:229-230, 355-365 (device -> host copy)
  *   od_get_state_async   solver.rs:757-759  (event + io_stream D2H)
  *   od_wait              solver.rs:785      (io_stream.synchronize in the IO task)
+ *   od_release           solver.rs:816      (done.send(buf): the buffer back to the pool)
  *   od_step              solver.rs:262-317  (RK3 loop: launch_rhs_stage +
  *                        launch_rk_stage x3, buffer swap) -> one hipGraph per step
  *   od_synchronize       solver.rs:334, 343 (stream.synchronize)
@@ -81,12 +82,19 @@ void od_destroy(od_ctx* ctx);
 od_status od_set_state(od_ctx* ctx, const void* host, size_t n_values, od_real dtype);
 od_status od_get_state(od_ctx* ctx, void* host, size_t n_values, od_real dtype);
 
-/* Snapshot the current state into library-owned pinned memory on a second
- * stream ordered after the work enqueued so far; od_wait(ticket) blocks until
- * the copy is complete and returns the pinned pointer (valid until the next
- * od_get_state_async that reuses the slot; 3 slots rotate). */
+/* Loan-and-return pool of 3 library-owned pinned staging slots
+ * (solver.rs:163-175, 320, 816).
+ * od_get_state_async borrows a free slot -- blocking until od_release
+ * returns one if all 3 are out -- and snapshots the current state into it on
+ * a second stream ordered after the work enqueued so far.  od_wait(ticket)
+ * blocks until the copy is complete and returns the pinned pointer, which
+ * stays valid until od_release(ticket) hands the slot back.  Every ticket
+ * must be released exactly once (also after a failed write), or a later
+ * od_get_state_async blocks for ever.  od_wait / od_release may be called
+ * from any thread; od_get_state_async from the thread that calls od_step. */
 od_status od_get_state_async(od_ctx* ctx, uint64_t* ticket);
 od_status od_wait(od_ctx* ctx, uint64_t ticket, const void** host_state);
+od_status od_release(od_ctx* ctx, uint64_t ticket);
 
 /* Enqueue n_steps RK3 steps of size dt (the f32 config value widened).
  * Asynchronous; replays of a captured 8-step hipGraph plus a 1-step one

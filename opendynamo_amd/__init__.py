@@ -16,10 +16,19 @@ OD_F32, OD_F64 = 4, 8
 FLUX = {"slau2": 0, "ausm+": 1, "rusanov": 2}
 _NP = {OD_F32: np.float32, OD_F64: np.float64}
 
+
+def _state_dtype(default, dtype):
+    """float32 or float64 (the only types the C side writes); anything else
+    is refused instead of letting 8-byte values overrun a smaller buffer."""
+    dt = default if dtype is None else np.dtype(dtype).type
+    if dt not in (np.float32, np.float64):
+        raise ValueError(f"state dtype must be float32 or float64, not {np.dtype(dt).name}")
+    return dt
+
 # Every symbol include/opendynamo.h declares.
 EXPORTS = [
     "od_default_params", "od_create", "od_destroy", "od_set_state", "od_get_state",
-    "od_get_state_async", "od_wait", "od_step", "od_synchronize", "od_rhs",
+    "od_get_state_async", "od_wait", "od_release", "od_step", "od_synchronize", "od_rhs",
     "od_conservation_sums", "od_cfl_dt", "od_time_steps", "od_time_kernels", "od_info",
     "od_last_error",
     "od_batch_create", "od_batch_destroy", "od_batch_set_state", "od_batch_get_state",
@@ -98,6 +107,7 @@ def lib():
         L.od_get_state.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
         L.od_get_state_async.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
         L.od_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.od_release.argtypes = [vp, ctypes.c_uint64]
         L.od_step.argtypes = [vp, ctypes.c_double, ctypes.c_uint32]
         L.od_synchronize.argtypes = [vp]
         L.od_rhs.argtypes = [vp, vp, ctypes.c_size_t]
@@ -218,7 +228,7 @@ class Solver:
         _check(lib().od_set_state(self._h, a.ctypes.data, a.size, code), "od_set_state")
 
     def get_state(self, dtype=None):
-        dt = self.dtype if dtype is None else np.dtype(dtype).type
+        dt = _state_dtype(self.dtype, dtype)
         out = np.empty(self.n_values, dtype=dt)
         code = OD_F32 if dt == np.float32 else OD_F64
         _check(lib().od_get_state(self._h, out.ctypes.data, out.size, code), "od_get_state")
@@ -265,11 +275,19 @@ class Solver:
         _check(lib().od_get_state_async(self._h, ctypes.byref(t)), "od_get_state_async")
         return t.value
 
-    def wait(self, ticket):
+    def wait(self, ticket, release=True):
+        """Copy of the snapshot of `ticket`; the pinned slot goes back to the
+        pool afterwards unless release=False (then call release(ticket))."""
         ptr = ctypes.c_void_p()
         _check(lib().od_wait(self._h, ctypes.c_uint64(ticket), ctypes.byref(ptr)), "od_wait")
         buf = (ctypes.c_char * (self.n_values * self.p.u_real)).from_address(ptr.value)
-        return np.frombuffer(buf, dtype=self.dtype).copy()
+        out = np.frombuffer(buf, dtype=self.dtype).copy()
+        if release:
+            self.release(ticket)
+        return out
+
+    def release(self, ticket):
+        _check(lib().od_release(self._h, ctypes.c_uint64(ticket)), "od_release")
 
     def info(self):
         out = np.zeros(4)
@@ -377,7 +395,7 @@ class Batch:
         _check(lib().od_batch_set_state(self._h, int(box), a.ctypes.data, a.size, code), "od_batch_set_state")
 
     def get_state(self, box, dtype=None):
-        dt = self.dtype if dtype is None else np.dtype(dtype).type
+        dt = _state_dtype(self.dtype, dtype)
         out = np.empty(self.n_values(box), dtype=dt)
         code = OD_F32 if dt == np.float32 else OD_F64
         _check(lib().od_batch_get_state(self._h, int(box), out.ctypes.data, out.size, code), "od_batch_get_state")

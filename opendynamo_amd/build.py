@@ -65,9 +65,13 @@ def build_host():
         return None
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HDF5_PREFIX, "include")]
     lib = os.path.join(HDF5_PREFIX, "lib")
-    # libhdf5 1.10 and libblosc (with lz4 / zstd) from the image's conda prefix
-    # the system libstdc++ must win over the older one in the conda prefix
-    link = ["-L" + lib, "-Wl,-rpath,/usr/lib/x86_64-linux-gnu:" + lib, "-lhdf5", "-lblosc"]
+    # libhdf5 1.10 and libblosc (with lz4 / zstd) from the image's conda prefix.
+    # The system libstdc++ must win over the older one in the conda prefix, at
+    # link time (the compiler's own directory ahead of the prefix) and at run
+    # time (the system directory first in the rpath).
+    gcc_dir = os.path.dirname(subprocess.run(["g++", "-print-file-name=libstdc++.so"], capture_output=True,
+                                             text=True, check=True).stdout.strip())
+    link = ["-L" + gcc_dir, "-L" + lib, "-Wl,-rpath,/usr/lib/x86_64-linux-gnu:" + lib, "-lhdf5", "-lblosc"]
     _run(["g++", *CXX_FLAGS, "-shared", *inc, "-o", HOSTLIB + ".tmp", *srcs, *link])
     os.replace(HOSTLIB + ".tmp", HOSTLIB)
     main = os.path.join(HOST, "main.cpp")

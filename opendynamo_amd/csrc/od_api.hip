@@ -379,9 +379,19 @@ od_status od_get_state(od_ctx* c, void* host, size_t n, od_real dtype) {
 od_status od_get_state_async(od_ctx* c, uint64_t* ticket) {
     if (!c || !ticket) return fail(OD_ERR_INVALID, "od_get_state_async: null argument");
     HIP_TRY(hipSetDevice(c->device));
-    const uint64_t t = c->next_ticket++;
-    const int slot = (int)(t % 3);
-    // the slot's previous copy must be complete before it is overwritten
+    // Borrow a free pinned slot, blocking until a consumer returns one
+    // (pool_rx.recv(), solver.rs:320): a slot is never refilled while its
+    // previous snapshot is still being read.
+    int slot = -1;
+    uint64_t t = 0;
+    {
+        std::unique_lock<std::mutex> lk(c->io_mu);
+        c->io_cv.wait(lk, [c] { return !c->slot_ticket[0] || !c->slot_ticket[1] || !c->slot_ticket[2]; });
+        slot = !c->slot_ticket[0] ? 0 : (!c->slot_ticket[1] ? 1 : 2);
+        t = c->next_ticket++;
+        c->slot_ticket[slot] = t;
+    }
+    // a consumer may return the slot without waiting for its copy
     HIP_TRY(hipEventSynchronize(c->slot_done[slot]));
     hipEvent_t ready;
     HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
@@ -391,18 +401,41 @@ od_status od_get_state_async(od_ctx* c, uint64_t* ticket) {
                            c->io_stream));
     HIP_TRY(hipEventRecord(c->slot_done[slot], c->io_stream));
     HIP_TRY(hipEventDestroy(ready));
-    c->slot_ticket[slot] = t;
     *ticket = t;
     return OD_OK;
 }
 
+namespace {
+// Slot loaned to `ticket`, or -1.  Caller holds io_mu.
+int loaned_slot(const od_ctx* c, uint64_t ticket) {
+    for (int q = 0; q < 3; ++q)
+        if (ticket != 0 && c->slot_ticket[q] == ticket) return q;
+    return -1;
+}
+}  // namespace
+
 od_status od_wait(od_ctx* c, uint64_t ticket, const void** host_state) {
     if (!c) return fail(OD_ERR_INVALID, "od_wait: null context");
-    const int slot = (int)(ticket % 3);
-    if (ticket == 0 || c->slot_ticket[slot] != ticket)
-        return fail(OD_ERR_STATE, "od_wait: ticket expired (slot reused) or unknown");
+    int slot;
+    {
+        std::lock_guard<std::mutex> lk(c->io_mu);
+        slot = loaned_slot(c, ticket);
+    }
+    if (slot < 0) return fail(OD_ERR_STATE, "od_wait: unknown or already released ticket");
     HIP_TRY(hipEventSynchronize(c->slot_done[slot]));
     if (host_state) *host_state = c->pinned[slot];
+    return OD_OK;
+}
+
+od_status od_release(od_ctx* c, uint64_t ticket) {
+    if (!c) return fail(OD_ERR_INVALID, "od_release: null context");
+    {
+        std::lock_guard<std::mutex> lk(c->io_mu);
+        const int slot = loaned_slot(c, ticket);
+        if (slot < 0) return fail(OD_ERR_STATE, "od_release: unknown or already released ticket");
+        c->slot_ticket[slot] = 0;
+    }
+    c->io_cv.notify_all();
     return OD_OK;
 }
 

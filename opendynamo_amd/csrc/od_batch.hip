@@ -547,16 +547,32 @@ od_status od_batch_autotune(od_batch* b, const double* dt, uint32_t steps_per_tr
                 for (int g : gpcs) cand.push_back(od_schedule{orient, pairs, order, g});
     }
     const size_t state_bytes = (size_t)5 * 64 * b->E * b->ubytes;
+    const od_schedule orig = b->sched;
     void* bak = nullptr;
+    bool backed_up = false;
     if (hipMalloc(&bak, state_bytes) != hipSuccess) return fail(OD_ERR_NOMEM, "od_batch_autotune: backup allocation failed");
+    // On failure the batch is left as it was: the state from before the
+    // trials (and its traces) and the original schedule; the first error's
+    // message is kept.
     auto done = [&](od_status s) {
         (void)hipStreamSynchronize(b->stream);
+        if (s != OD_OK && backed_up) {
+            const std::string msg = od_last_error();
+            if (std::memcmp(&b->sched, &orig, sizeof(orig)) != 0) {
+                b->sched = orig;
+                drop_graph(b);
+            }
+            (void)hipMemcpyAsync(b->u[0], bak, state_bytes, hipMemcpyDeviceToDevice, b->stream);
+            (void)retrace_all(b);
+            (void)hipStreamSynchronize(b->stream);
+            s = fail(s, msg);
+        }
         (void)hipFree(bak);
         return s;
     };
     if (hipMemcpyAsync(bak, b->u[0], state_bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
         return done(fail(OD_ERR_HIP, "od_batch_autotune: backup copy failed"));
-    const od_schedule orig = b->sched;
+    backed_up = true;
     od_schedule win = orig;
     double win_ms = 1e300;
     int32_t count = 0;

@@ -3,6 +3,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <condition_variable>
+#include <mutex>
+
 #include "od_host.hpp"
 
 struct od_ctx {
@@ -24,8 +27,13 @@ struct od_ctx {
     double graph_dt = 0.0;
     void* pinned[3] = {nullptr, nullptr, nullptr};
     hipEvent_t slot_done[3] = {nullptr, nullptr, nullptr};
+    // Pinned staging pool (solver.rs:163-175): a slot is loaned to a ticket by
+    // od_get_state_async and returned by od_release; slot_ticket 0 = free.
+    // Guarded by io_mu; io_cv signals a returned slot.
     uint64_t next_ticket = 1;
     uint64_t slot_ticket[3] = {0, 0, 0};
+    std::mutex io_mu;
+    std::condition_variable io_cv;
     size_t device_bytes = 0;
     // learned closure: per-element eddy viscosity [E] in the compute type
     void* mu_t = nullptr;

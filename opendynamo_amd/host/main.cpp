@@ -3,10 +3,13 @@
 // argv[1]), loads [physics].initial_state, saves the startup state, advances
 // n_steps RK3 steps on the GPU through libopendynamo_hip, and writes dumps and
 // checkpoints from background IO threads with the reference's labels.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -25,6 +28,59 @@ void od_check(od_status s, const char* what) {
 const char* flux_debug_name(const std::string& s) {  // main.rs:17 prints the enum
     return s == "slau2" ? "Slau2" : s == "ausm+" ? "AusmPlus" : "Rusanov";
 }
+
+// Background writers of the dumps (spawn_io_task, solver.rs:745-818).  Each
+// holds its pinned slot from od_wait until the file is written and then
+// returns it with od_release, on every path, so the step loop's next
+// od_get_state_async never refills a buffer that is still being read.  The
+// destructor joins every writer, so writers finish before the context they
+// use is destroyed, also when an exception leaves the step loop.
+class WriterPool {
+   public:
+    template <class F>
+    void spawn(od_ctx* ctx, uint64_t ticket, F write) {
+        reap();
+        auto done = std::make_shared<std::atomic<bool>>(false);
+        writers_.push_back({std::thread([ctx, ticket, write, done]() {
+                                try {
+                                    const void* host = nullptr;
+                                    od_check(od_wait(ctx, ticket, &host), "od_wait");
+                                    write(host);
+                                } catch (const std::exception& e) {
+                                    std::fprintf(stderr, "IO task failed: %s\n", e.what());
+                                }
+                                if (od_release(ctx, ticket) != OD_OK)
+                                    std::fprintf(stderr, "IO task failed: od_release: %s\n", od_last_error());
+                                done->store(true, std::memory_order_release);
+                            }),
+                            done});
+    }
+    void join_all() {
+        for (auto& w : writers_)
+            if (w.th.joinable()) w.th.join();
+        writers_.clear();
+    }
+    ~WriterPool() { join_all(); }
+
+   private:
+    struct Writer {
+        std::thread th;
+        std::shared_ptr<std::atomic<bool>> done;
+    };
+    // join the writers that have finished; at most 3 are unfinished at a time
+    // (each holds a slot), so the list stays short
+    void reap() {
+        for (auto it = writers_.begin(); it != writers_.end();) {
+            if (it->done->load(std::memory_order_acquire)) {
+                it->th.join();
+                it = writers_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    std::deque<Writer> writers_;
+};
 
 // solver.rs:820-839: min/max of rho and u over the final state, in f32.
 template <class T>
@@ -111,37 +167,29 @@ int run(const std::string& cfg_path) {
         std::printf("dt from CFL %.3g: %.9g\n", *cfg.solver.cfl, (double)dt);
     }
     const uint64_t n_steps = cfg.solver.n_steps;
-    std::thread slot_thread[3];
+    WriterPool writers;  // declared after `destroy`: joined before the context goes
     std::printf("Starting RK3 explicit time integration...\n");
     const auto t0 = std::chrono::steady_clock::now();
     for (uint64_t step = 0; step < n_steps; ++step) {
         od_check(od_step(ctx, (double)dt, 1), "od_step");
         if (step % cfg.io.save_freq == 0 || step == n_steps - 1) {
             uint64_t ticket = 0;
+            // blocks while all 3 pinned slots are still held by writers
             od_check(od_get_state_async(ctx, &ticket), "od_get_state_async");
-            std::thread& slot = slot_thread[ticket % 3];
-            if (slot.joinable()) slot.join();  // the pinned slot is free once its writer is done
             const bool want_ckpt = step == n_steps - 1 || (cfg.io.checkpoint_freq > 0 && step % cfg.io.checkpoint_freq == 0);
-            slot = std::thread([=, &cfg]() {
-                try {
-                    const void* host = nullptr;
-                    od_check(od_wait(ctx, ticket, &host), "od_wait");
-                    // labels count completed steps; t = base_t + (k+1) dt (solver.rs:769-810)
-                    const uint64_t label = base_step + step + 1;
-                    const double t = base_t + ((double)step + 1.0) * (double)dt;
-                    odh::save_state_h5(label, t, kind, cfg.io.output_dir, cfg.io.prefix, host, cfg.mesh.nx,
+            // labels count completed steps; t = base_t + (k+1) dt (solver.rs:769-810)
+            const uint64_t label = base_step + step + 1;
+            const double t = base_t + ((double)step + 1.0) * (double)dt;
+            writers.spawn(ctx, ticket, [=, &cfg](const void* host) {
+                odh::save_state_h5(label, t, kind, cfg.io.output_dir, cfg.io.prefix, host, cfg.mesh.nx, cfg.mesh.ny,
+                                   cfg.mesh.nz, lx, ly, lz, comp);
+                if (want_ckpt)
+                    odh::save_state_h5(label, t, kind, cfg.io.output_dir, "checkpoint", host, cfg.mesh.nx,
                                        cfg.mesh.ny, cfg.mesh.nz, lx, ly, lz, comp);
-                    if (want_ckpt)
-                        odh::save_state_h5(label, t, kind, cfg.io.output_dir, "checkpoint", host, cfg.mesh.nx,
-                                           cfg.mesh.ny, cfg.mesh.nz, lx, ly, lz, comp);
-                } catch (const std::exception& e) {
-                    std::fprintf(stderr, "IO task failed: %s\n", e.what());
-                }
             });
         }
     }
-    for (auto& t : slot_thread)
-        if (t.joinable()) t.join();
+    writers.join_all();
     od_check(od_synchronize(ctx), "od_synchronize");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("Simulation finished in %.3f seconds.\n", secs);

@@ -109,3 +109,65 @@ def test_cli_matches_library(tmp_path):
         s.step(0.002, 6)
         ref = s.get_state()
     assert got.tobytes() == ref.tobytes()
+
+
+def test_cli_every_dump_matches_library_with_slow_writes(tmp_path):
+    """save_freq = 1 with gzip level 9 at 16^3: writes take far longer than a
+    step, so all three pinned slots are out at once and the step loop must
+    wait for a writer to hand one back (solver.rs:320, 816).  Every dump must
+    equal the library state after that many steps, byte for byte."""
+    n, nx = 12, 16
+    out = tmp_path / "slow"
+    out.mkdir()
+    cfg = out / "config.toml"
+    text = TEMPLATE.format(init=str(out / "flow_f32_000000.h5"), n=n, out=out, real="f32", fast="true")
+    text = text.replace("nx = 4\nny = 4\nnz = 3", f"nx = {nx}\nny = {nx}\nnz = {nx}")
+    text = text.replace("save_freq = 2\ncheckpoint_freq = 4\ncompression = \"blosc-lz4\"",
+                        "save_freq = 1\ncheckpoint_freq = 0\ncompression = \"gzip9\"")
+    assert "save_freq = 1" in text and f"nz = {nx}" in text
+    cfg.write_text(text)
+    init = make_initial_state.main([str(cfg)])
+    run_cli(str(cfg))
+    labels = sorted(int(f[-9:-3]) for f in os.listdir(out) if f.startswith("flow_f32_"))
+    assert labels == list(range(n + 1))
+    u0, _, _ = host.load_state(init, nx, nx, nx, "f32")
+    p = od.params(nx, nx, nx, 6.283185307179586, 2.0, 3.141592653589793, u_real=4, flux="slau2", y_wall=True,
+                  viscosity_model=1, mu_ref=0.001, fast_math=1)
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        for k in range(1, n + 1):
+            s.step(0.002, 1)
+            ref = s.get_state()
+            got, step, t = host.load_state(str(out / f"flow_f32_{k:06d}.h5"), nx, nx, nx, "f32")
+            assert step == k and t == k * float(np.float32(0.002))
+            assert got.tobytes() == ref.tobytes(), f"dump {k} differs from the library state"
+
+
+def test_async_pool_loan_and_return():
+    """Three slots: a fourth snapshot needs a returned slot; a released
+    ticket is refused; each snapshot is the state at its own call."""
+    p = od.params(2, 2, 2, 6.283185307179586, 6.283185307179586, 6.283185307179586, u_real=8, flux="rusanov",
+                  viscous=False)
+    from helpers import smooth_state
+    u = smooth_state(2, 2, 2, 6.283185307179586, 6.283185307179586, 6.283185307179586, seed=1)
+    print("CASE async pool", flush=True)
+    with od.Solver(p) as s:
+        s.set_state(u)
+        tickets, refs = [], []
+        for _ in range(3):
+            s.step(1e-3, 1)
+            tickets.append(s.get_state_async())
+            refs.append(s.get_state())
+        snaps = [s.wait(t, release=False) for t in tickets]
+        for a, b in zip(snaps, refs):
+            assert a.tobytes() == b.tobytes()
+        s.release(tickets[1])
+        with pytest.raises(RuntimeError):
+            s.release(tickets[1])
+        with pytest.raises(RuntimeError):
+            s.wait(tickets[1])
+        s.step(1e-3, 1)
+        t4 = s.get_state_async()          # takes the returned slot
+        assert s.wait(t4).tobytes() == s.get_state().tobytes()
+        s.release(tickets[0])
+        s.release(tickets[2])
