@@ -85,8 +85,10 @@ od_status od_get_state(od_ctx* ctx, void* host, size_t n_values, od_real dtype);
 /* Loan-and-return pool of 3 library-owned pinned staging slots
  * (solver.rs:163-175, 320, 816).
  * od_get_state_async borrows a free slot -- blocking until od_release
- * returns one if all 3 are out -- and snapshots the current state into it on
- * a second stream ordered after the work enqueued so far.  od_wait(ticket)
+ * returns one if all 3 are out -- and snapshots the current state into it:
+ * a device-to-device copy in the compute stream's order (so steps enqueued
+ * later cannot change the snapshot), then the copy to the pinned slot on a
+ * second stream, overlapping later steps.  od_wait(ticket)
  * blocks until the copy is complete and returns the pinned pointer, which
  * stays valid until od_release(ticket) hands the slot back.  Every ticket
  * must be released exactly once (also after a failed write), or a later

@@ -34,6 +34,12 @@ HDF5_PREFIX = os.environ.get("OD_HDF5_PREFIX", "/opt/conda")
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-ffp-contract=off", "-pthread", "-Wall"]
 
 
+def fast_flags(flags):
+    """Flags of the fast-math translation unit: contraction allowed, except
+    where the source narrows it with #pragma clang fp contract."""
+    return [f for f in flags if f != "-ffp-contract=off"] + ["-ffp-contract=fast-honor-pragmas"]
+
+
 def _run(cmd):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -49,8 +55,8 @@ def build_lib():
     jobs = []
     for s, o in zip(srcs, objs):
         flags = list(HIP_FLAGS)
-        if s == "od_kernels_fast.hip":  # fast-math build: contraction allowed
-            flags = [f for f in flags if f != "-ffp-contract=off"] + ["-ffp-contract=fast"]
+        if s == "od_kernels_fast.hip":
+            flags = fast_flags(flags)
         jobs.append([HIPCC, *flags, "-c", os.path.join(CSRC, s), "-o", o])
     with cf.ThreadPoolExecutor(len(jobs)) as ex:
         list(ex.map(_run, jobs))

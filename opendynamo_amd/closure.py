@@ -247,7 +247,7 @@ DEFAULT = dict(L=TWO_PI, mach=0.1, re=400.0, n_modes=64, amplitude=0.1, kmax=4, 
                mid=32, snap_times=[5.0, 10.0, 15.0, 20.0, 25.0, 30.0, 35.0, 40.0], eval_time=5.0, run_time=40.0,
                train_seeds=list(range(8)), eval_seeds=[100, 101], epochs=300, batch=65536, lr=2e-3,
                update_interval=5, budget_kappa=1.0, calib_seeds=[0, 1], multipliers=[1, 2, 4, 8, 16, 32],
-               calib_objective="all_times")
+               constant_multipliers=[1, 2, 4, 6, 8, 11, 16, 23, 32, 45, 64], calib_objective="all_times")
 
 
 def viscosity(cfg):
@@ -312,14 +312,40 @@ def train(cfg, X, T, device=0, log=print):
                    r2_train=r2_train, r2_heldout=r2_val)
 
 
+def constant_model(base_scale, shift, scale, lo, hi, device=0):
+    """The constant-coefficient eddy viscosity in the closure's own form: a
+    network with zero weights whose output bias makes softplus(.) = 1, so
+    C = s for every element and mu_t = rho_e Delta^2 |S|_e s (a Smagorinsky
+    type model).  It runs through the same kernels, training-range fallback,
+    budget check and multiplier calibration as the trained network, which
+    makes it the baseline that shows what the network's element-local
+    variation of C adds."""
+    params = np.zeros(od.MLP_PARAMS, np.float32)
+    params[-1] = np.log(np.expm1(1.0))  # softplus(b4) = 1
+    m = od.MLP(device, params=params)
+    m.set_scaling(shift, scale, lo, hi)
+    m.set_output_scale(base_scale)
+    return m
+
+
 def check_times(cfg):
     return [5.0 * q for q in range(1, int(cfg["run_time"] / 5.0) + 1)]
 
 
+_FINE_CACHE = {}
+
+
 def fine_spectra(cfg, seed, times):
+    """Spectra of the fine run at `times` (kept for the process: calibration
+    and evaluation of several models share the fine runs of a seed)."""
     nf = cfg["fine"]
-    fine = run(nf, cfg["L"], viscosity(cfg), seed, cfg, times)
-    return [energy_spectrum(uniform_velocity(st, nf), cfg["L"]) for st in fine["states"]], fine
+    key = (nf, cfg["L"], viscosity(cfg), seed, cfg["n_modes"], cfg["amplitude"], cfg["kmax"], cfg["flux"],
+           tuple(times))
+    if key not in _FINE_CACHE:
+        fine = run(nf, cfg["L"], viscosity(cfg), seed, cfg, times)
+        specs = [energy_spectrum(uniform_velocity(st, nf), cfg["L"]) for st in fine["states"]]
+        _FINE_CACHE[key] = (specs, dict(wall=fine["wall"], steps=fine["steps"]))
+    return _FINE_CACHE[key]
 
 
 def coarse_errors(cfg, n, seed, times, E_refs, k_max, mlp=None):
@@ -342,7 +368,7 @@ def coarse_errors(cfg, n, seed, times, E_refs, k_max, mlp=None):
                 stats=res["stats"])
 
 
-def calibrate(cfg, mlp, base_scale, log=print):
+def calibrate(cfg, mlp, base_scale, log=print, multipliers=None):
     """A-posteriori multiplier of the output scale (see the module docstring).
     Returns the chosen multiplier and the trial table."""
     nc = cfg["fine"] // cfg["ratio"]
@@ -352,7 +378,7 @@ def calibrate(cfg, mlp, base_scale, log=print):
     refs = {seed: fine_spectra(cfg, seed, times)[0] for seed in cfg["calib_seeds"]}
     table = []
     key = "err_all_times" if cfg.get("calib_objective", "eval_time") == "all_times" else "err_eval_time"
-    for m in cfg["multipliers"]:
+    for m in (multipliers or cfg["multipliers"]):
         mlp.set_output_scale(base_scale * m)
         rows = [coarse_errors(cfg, nc, seed, times, refs[seed], k_max, mlp) for seed in cfg["calib_seeds"]]
         stable = all(r["finite_until"] >= cfg["run_time"] for r in rows)
@@ -373,7 +399,7 @@ def calibrate(cfg, mlp, base_scale, log=print):
     return best["multiplier"], table
 
 
-def evaluate(cfg, mlp, log=print):
+def evaluate(cfg, mlp, log=print, constant=None):
     nf, r, nm = cfg["fine"], cfg["ratio"], cfg["mid"]
     nc = nf // r
     k_max = (nc * 4) // 3  # 2/3 of the coarse grid's Nyquist wavenumber (4 nc / 2)
@@ -383,7 +409,10 @@ def evaluate(cfg, mlp, log=print):
     for seed in cfg["eval_seeds"]:
         E_refs, fine = fine_spectra(cfg, seed, times)
         out = {"seed": seed, "times": times, "fine_wall_s": fine["wall"], "fine_steps": fine["steps"]}
-        for name, n, net in (("coarse_closure", nc, mlp), ("coarse_plain", nc, None), ("mid_plain", nm, None)):
+        models = [("coarse_closure", nc, mlp), ("coarse_plain", nc, None), ("mid_plain", nm, None)]
+        if constant is not None:
+            models.insert(1, ("coarse_constant", nc, constant))
+        for name, n, net in models:
             row = coarse_errors(cfg, n, seed, times, E_refs, k_max, net)
             row["err"] = row["errs"][te]  # the evaluation time
             row["spectrum"] = row["spectra"][te]
@@ -416,6 +445,9 @@ def main(argv=None):
                     help="calibrate on the mean spectral error over every check time, or at the evaluation time")
     ap.add_argument("--multiplier", type=float, default=None,
                     help="output-scale multiplier (default: calibrated a posteriori on the calibration seeds)")
+    ap.add_argument("--constant-multipliers", type=float, nargs="+", default=DEFAULT["constant_multipliers"],
+                    help="candidate multipliers of the constant-coefficient baseline (calibrated the same way)")
+    ap.add_argument("--no-constant", action="store_true", help="skip the constant-coefficient baseline")
     args = ap.parse_args(argv)
     cfg = dict(DEFAULT, fine=args.fine, ratio=args.ratio, mid=args.mid, train_seeds=list(range(args.seeds)),
                epochs=args.epochs, snap_times=args.snap_times, eval_time=args.eval_time, flux=args.flux,
@@ -456,11 +488,17 @@ def main(argv=None):
         report["calibration"] = dict(multiplier=mult, trials=None)
     # out_scale is the trained scale; the closure runs with out_scale * output_multiplier
     np.savez(os.path.join(args.out, "closure_weights.npz"), output_multiplier=mult, **wts)
-    report["evaluation"] = evaluate(cfg, mlp, log)
+    const = None
+    if not args.no_constant:
+        const = constant_model(base, wts["shift"], wts["scale"], wts["lo"], wts["hi"])
+        cmult, ctable = calibrate(cfg, const, base, log, multipliers=list(args.constant_multipliers))
+        report["constant_baseline"] = dict(multiplier=cmult, trials=ctable)
+        log(f"constant-coefficient baseline: multiplier x{cmult}")
+    report["evaluation"] = evaluate(cfg, mlp, log, constant=const)
     with open(os.path.join(args.out, "closure_report.json"), "w") as f:
         json.dump(report, f)
-    log(json.dumps([{k: r[k]["err"] for k in ("coarse_closure", "coarse_plain", "mid_plain")}
-                    for r in report["evaluation"]["runs"]]))
+    names = ("coarse_closure", "coarse_constant", "coarse_plain", "mid_plain")
+    log(json.dumps([{k: r[k]["err"] for k in names if k in r} for r in report["evaluation"]["runs"]]))
 
 
 if __name__ == "__main__":

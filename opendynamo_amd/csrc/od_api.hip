@@ -147,6 +147,12 @@ int launch_kind(od_ctx* c, int kind, int stage, double dt, int mode) {
     }
     if constexpr (sizeof(CT) == 4) {
         if (c->p.fast_math) {
+            if constexpr (sizeof(UT) == 4) {
+                if (kind == 0 && c->blocks) return od::fastmath::launch_visc_block(a, c->stream);
+                if (kind == 1 && c->blocks)
+                    return c->p.viscous ? od::fastmath::launch_update_block<true>(a, true, c->stream)
+                                        : od::fastmath::launch_update_block<false>(a, true, c->stream);
+            }
             if (kind == 0) return od::fastmath::launch_visc<UT, CT>(a, c->E, c->stream);
             if (c->p.viscous) return od::fastmath::launch_update<UT, CT, true>(a, c->E, c->stream);
             return od::fastmath::launch_update<UT, CT, false>(a, c->E, c->stream);
@@ -270,6 +276,9 @@ od_status od_create(const od_params* p, int device, od_ctx** out) {
     c->NP = E * 64;
     c->ubytes = (size_t)p->u_real;
     c->cbytes = (size_t)p->compute_real;
+    // fast f32 with even dimensions: the step kernels on 2x2x2-element blocks
+    c->blocks = p->fast_math && p->u_real == OD_F32 && p->compute_real == OD_F32 && p->nx % 2 == 0 &&
+                p->ny % 2 == 0 && p->nz % 2 == 0;
     c->ops = fr_operators();
     {
         const od::Ops<float> of = cast_ops<float>(c->ops);
@@ -327,6 +336,8 @@ void od_destroy(od_ctx* c) {
     for (int q = 0; q < 3; ++q) {
         if (c->pinned[q]) (void)hipHostFree(c->pinned[q]);
         if (c->slot_done[q]) (void)hipEventDestroy(c->slot_done[q]);
+        if (c->staged[q]) (void)hipFree(c->staged[q]);
+        if (c->staged_done[q]) (void)hipEventDestroy(c->staged_done[q]);
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->io_stream) (void)hipStreamDestroy(c->io_stream);
@@ -391,16 +402,40 @@ od_status od_get_state_async(od_ctx* c, uint64_t* ticket) {
         t = c->next_ticket++;
         c->slot_ticket[slot] = t;
     }
-    // a consumer may return the slot without waiting for its copy
-    HIP_TRY(hipEventSynchronize(c->slot_done[slot]));
-    hipEvent_t ready;
-    HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(ready, c->stream));
-    HIP_TRY(hipStreamWaitEvent(c->io_stream, ready, 0));
-    HIP_TRY(hipMemcpyAsync(c->pinned[slot], c->u[0], (size_t)5 * c->NP * c->ubytes, hipMemcpyDeviceToHost,
-                           c->io_stream));
-    HIP_TRY(hipEventRecord(c->slot_done[slot], c->io_stream));
-    HIP_TRY(hipEventDestroy(ready));
+    // On an error below the slot goes back to the pool before returning.
+    auto give_back = [c, slot](od_status st) {
+        {
+            std::lock_guard<std::mutex> lk(c->io_mu);
+            c->slot_ticket[slot] = 0;
+        }
+        c->io_cv.notify_all();
+        return st;
+    };
+    const size_t bytes = (size_t)5 * c->NP * c->ubytes;
+    if (!c->staged[slot]) {
+        if (hipMalloc(&c->staged[slot], bytes) != hipSuccess) {
+            c->staged[slot] = nullptr;
+            return give_back(fail(OD_ERR_NOMEM, "od_get_state_async: snapshot buffer allocation failed"));
+        }
+        c->device_bytes += bytes;
+        if (hipEventCreateWithFlags(&c->staged_done[slot], hipEventDisableTiming) != hipSuccess)
+            return give_back(fail(OD_ERR_HIP, "od_get_state_async: event creation failed"));
+    }
+    // a consumer may return the slot without waiting for its copy: the slot's
+    // previous device-to-host copy must be complete before either buffer is
+    // refilled
+    hipError_t e = hipEventSynchronize(c->slot_done[slot]);
+    // the snapshot in stream order on the compute stream (before the next
+    // step rewrites u[0]), then to the pinned slot on the IO stream
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->staged[slot], c->u[0], bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->staged_done[slot], c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->io_stream, c->staged_done[slot], 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->pinned[slot], c->staged[slot], bytes, hipMemcpyDeviceToHost, c->io_stream);
+    if (e == hipSuccess) e = hipEventRecord(c->slot_done[slot], c->io_stream);
+    if (e != hipSuccess)
+        return give_back(fail(OD_ERR_HIP, std::string("od_get_state_async: ") + hipGetErrorString(e)));
     *ticket = t;
     return OD_OK;
 }
@@ -580,7 +615,10 @@ od_status od_time_kernels(od_ctx* c, double dt, uint32_t n_steps, double out_ms[
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 4; ++k) {
         out_ms[k] = (k < kinds && cnt[k]) ? sum[k] / cnt[k] : 0.0;
-        if (names) names[k] = k < kinds ? od::kernel_name(k) : "";
+        const char* nm = k < kinds ? od::kernel_name(k) : "";
+        if (k == 0 && c->blocks) nm = "od_visc_block_kernel";
+        if (k == 1 && c->blocks) nm = "od_update_block_kernel";
+        if (names) names[k] = nm;
     }
     if (n_kinds) *n_kinds = kinds;
     return OD_OK;

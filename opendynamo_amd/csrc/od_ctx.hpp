@@ -22,11 +22,19 @@ struct od_ctx {
     double* sums_out = nullptr;
     od::host::HostOps ops;
     bool state_set = false;
+    bool blocks = false;               // step kernels on 2x2x2-element blocks (fast f32, even dims),
+                                       // which store the traces of the blocks' external faces only
     hipGraphExec_t graph = nullptr;    // one RK3 step
     hipGraphExec_t graph_k = nullptr;  // kStepsPerGraph RK3 steps
     double graph_dt = 0.0;
     void* pinned[3] = {nullptr, nullptr, nullptr};
     hipEvent_t slot_done[3] = {nullptr, nullptr, nullptr};
+    // Device copies of the snapshots (allocated at the first snapshot): the
+    // next step's last stage rewrites u[0] while the IO stream may still be
+    // copying, so the compute stream first copies u[0] here (device to
+    // device, in stream order) and the IO stream copies from here.
+    void* staged[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t staged_done[3] = {nullptr, nullptr, nullptr};
     // Pinned staging pool (solver.rs:163-175): a slot is loaned to a ticket by
     // od_get_state_async and returned by od_release; slot_ticket 0 = free.
     // Guarded by io_mu; io_cv signals a returned slot.

@@ -119,6 +119,13 @@ int launch_batch_update(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, 
 template <class UT, class CT>
 int batch_occupancy(int kind, int pairs, bool visc, int* groups_per_cu);
 int upload_ops(const Ops<float>& f32, const Ops<double>& f64);
+// Kernels on blocks of 2x2x2 elements (f32 state and compute, nx, ny, nz
+// even): the same results as launch_update / launch_visc, bit for bit.
+// ext_traces: the update kernel writes the traces of the blocks' external
+// faces only, which is all the block visc kernel reads.
+template <bool VISC>
+int launch_update_block(const StageArgs<float, float>& a, bool ext_traces, void* stream);
+int launch_visc_block(const StageArgs<float, float>& a, void* stream);
 }  // namespace fastmath
 
 }  // namespace od

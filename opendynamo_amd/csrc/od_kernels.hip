@@ -77,6 +77,19 @@ __device__ __forceinline__ C sq(C x) {
     return sqrt(x);
 }
 
+// Fast build: the arithmetic of the point and interface fluxes may fuse a
+// multiply into an add only within one source expression (as the C standard
+// allows), not across statements.  Fusion across statements is decided per
+// kernel from the shape of the whole computation (for a*b + c*d, which
+// product to fuse depends on how many other expressions use each), so the
+// same flux could round differently in the two-element-tile and the block
+// kernels; within an expression the choice is fixed by the source.
+#if OD_FAST
+#define OD_CONTRACT_IN_EXPRESSIONS _Pragma("clang fp contract(on)")
+#else
+#define OD_CONTRACT_IN_EXPRESSIONS
+#endif
+
 template <class T>
 __device__ __forceinline__ T sel3(T a0, T a1, T a2, int d) {
     return d == 0 ? a0 : (d == 1 ? a1 : a2);
@@ -94,6 +107,14 @@ __device__ __forceinline__ C gamma_m1() { return ((C)1.4) - ((C)1.0); }
 
 template <class C>
 __device__ __forceinline__ C pressure(C rho, C ru, C rv, C rw, C e) {  // :1-5
+    OD_CONTRACT_IN_EXPRESSIONS
+#if OD_FAST
+    if constexpr (sizeof(C) == 4) {  // fast build: the fusions written out
+        const C h = ((C)0.5) * __builtin_fmaf(rw, rw, __builtin_fmaf(rv, rv, ru * ru));
+        const C p = gamma_m1<C>() * __builtin_fmaf(-h, __builtin_amdgcn_rcpf(rho), e);
+        return fmax(((C)1e-7), p);
+    }
+#endif
     C ke = dv(((C)0.5) * (ru * ru + rv * rv + rw * rw), rho);
     C p = gamma_m1<C>() * (e - ke);
     return fmax(((C)1e-7), p);
@@ -101,16 +122,14 @@ __device__ __forceinline__ C pressure(C rho, C ru, C rv, C rw, C e) {  // :1-5
 
 template <class C>
 __device__ __forceinline__ void euler_flux(int d, const C* s, C* f) {  // :7-21
+    OD_CONTRACT_IN_EXPRESSIONS
     C rho = s[0], e = s[4];
     C p = pressure(rho, s[1], s[2], s[3], e);
     C md = sel3(s[1], s[2], s[3], d);
     C ud = dv(md, rho);
     f[0] = md;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        C t = s[1 + c] * ud;
-        f[1 + c] = (c == d) ? (t + p) : t;
-    }
+    for (int c = 0; c < 3; ++c) f[1 + c] = (c == d) ? s[1 + c] * ud + p : s[1 + c] * ud;
     f[4] = ud * (e + p);
 }
 
@@ -181,6 +200,7 @@ __device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu
 
 template <class C>
 __device__ __forceinline__ C sound_speed(C p, C rho) {
+    OD_CONTRACT_IN_EXPRESSIONS
     return sq(fmax(((C)0.0), dv(((C)1.4) * p, fmax(((C)1e-7), rho))));
 }
 
@@ -190,6 +210,7 @@ __device__ __forceinline__ C sound_speed(C p, C rho) {
 // sign of zero results; the fast build drops the zero terms.
 template <int D, class C>
 __device__ __forceinline__ C nvel(C u, C v, C w) {
+    OD_CONTRACT_IN_EXPRESSIONS
 #if OD_FAST
     return D == 0 ? u : (D == 1 ? v : w);
 #else
@@ -199,6 +220,7 @@ __device__ __forceinline__ C nvel(C u, C v, C w) {
 }
 template <int D, class C>
 __device__ __forceinline__ C mflux(int c, C a, C b, C p) {
+    OD_CONTRACT_IN_EXPRESSIONS
 #if OD_FAST
     return c == D ? a * b + p : a * b;
 #else
@@ -208,6 +230,7 @@ __device__ __forceinline__ C mflux(int c, C a, C b, C p) {
 
 template <int D, class C>
 __device__ __forceinline__ void rusanov(const C* l, const C* r, C* f) {  // :104-128
+    OD_CONTRACT_IN_EXPRESSIONS
     C ul = dv(l[1], l[0]), vl = dv(l[2], l[0]), wl = dv(l[3], l[0]);
     C ur = dv(r[1], r[0]), vr = dv(r[2], r[0]), wr = dv(r[3], r[0]);
     C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
@@ -232,6 +255,7 @@ __device__ __forceinline__ void rusanov(const C* l, const C* r, C* f) {  // :104
 
 template <int D, class C>
 __device__ __forceinline__ void ausm_plus(const C* l, const C* r, C* f) {  // :130-188
+    OD_CONTRACT_IN_EXPRESSIONS
     C ul = dv(l[1], l[0]), vl = dv(l[2], l[0]), wl = dv(l[3], l[0]);
     C ur = dv(r[1], r[0]), vr = dv(r[2], r[0]), wr = dv(r[3], r[0]);
     C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
@@ -272,6 +296,7 @@ __device__ __forceinline__ void ausm_plus(const C* l, const C* r, C* f) {  // :1
 
 template <int D, class C>
 __device__ __forceinline__ void slau2(const C* l, const C* r, C* f) {  // :190-257
+    OD_CONTRACT_IN_EXPRESSIONS
     C ul = dv(l[1], l[0]), vl = dv(l[2], l[0]), wl = dv(l[3], l[0]);
     C ur = dv(r[1], r[0]), vr = dv(r[2], r[0]), wr = dv(r[3], r[0]);
     C pl = pressure(l[0], l[1], l[2], l[3], l[4]);
@@ -686,9 +711,11 @@ __device__ __forceinline__ void br1_jumps(const StageArgs<UT, CT>& a, const CT (
 
 // BR1-corrected gradient at the lane's point (rhs.cu:366-465) from the
 // element's LDS tile and the face jumps; Q round trip included.
+// J0..J5: the jump blocks [5][16] of faces xL, xR, yL, yR, zL, zR.
 template <class UT, class CT>
-__device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const CT (*sU)[64], const CT (*sJmp)[5][16],
-                                              int i, int j, int k, CT* g) {
+__device__ __forceinline__ void lane_gradient_faces(const StageArgs<UT, CT>& a, const CT (*sU)[64], const CT* J0,
+                                                    const CT* J1, const CT* J2, const CT* J3, const CT* J4,
+                                                    const CT* J5, int i, int j, int k, CT* g) {
     const CoefTable<CT>& o = coef_lds<CT>();
     const CT* Di = o.D[i];
     const CT* Dj = o.D[j];
@@ -709,17 +736,17 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
         for (int m = 0; m < 4; ++m) deta += Dj[m] * sU[v][yb + 4 * m];
 #if OD_FAST
         // one Jacobian factor per direction
-        CT gx = (dxi + sJmp[0][v][fpx] * gLi + sJmp[1][v][fpx] * gRi) * a.jx;
-        CT gy = (deta + sJmp[2][v][fpy] * gLj + sJmp[3][v][fpy] * gRj) * a.jy;
-        CT gz = (dz + sJmp[4][v][fpz] * gLk + sJmp[5][v][fpz] * gRk) * a.jz;
+        CT gx = (dxi + J0[v * 16 + fpx] * gLi + J1[v * 16 + fpx] * gRi) * a.jx;
+        CT gy = (deta + J2[v * 16 + fpy] * gLj + J3[v * 16 + fpy] * gRj) * a.jy;
+        CT gz = (dz + J4[v * 16 + fpz] * gLk + J5[v * 16 + fpz] * gRk) * a.jz;
 #else
         CT gx = dxi * a.jx, gy = deta * a.jy, gz = dz * a.jz;
-        gx += sJmp[0][v][fpx] * gLi * a.jx;
-        gx += sJmp[1][v][fpx] * gRi * a.jx;
-        gy += sJmp[2][v][fpy] * gLj * a.jy;
-        gy += sJmp[3][v][fpy] * gRj * a.jy;
-        gz += sJmp[4][v][fpz] * gLk * a.jz;
-        gz += sJmp[5][v][fpz] * gRk * a.jz;
+        gx += J0[v * 16 + fpx] * gLi * a.jx;
+        gx += J1[v * 16 + fpx] * gRi * a.jx;
+        gy += J2[v * 16 + fpy] * gLj * a.jy;
+        gy += J3[v * 16 + fpy] * gRj * a.jy;
+        gz += J4[v * 16 + fpz] * gLk * a.jz;
+        gz += J5[v * 16 + fpz] * gRk * a.jz;
 #endif
         if (a.round_q) {  // Q_REAL round trip (rhs.cu:457-461), narrows only when CT is double
             gx = (CT)(float)gx;
@@ -734,6 +761,13 @@ __device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const 
         g[3 * v + 1] = gy;
         g[3 * v + 2] = gz;
     }
+}
+
+template <class UT, class CT>
+__device__ __forceinline__ void lane_gradient(const StageArgs<UT, CT>& a, const CT (*sU)[64], const CT (*sJmp)[5][16],
+                                              int i, int j, int k, CT* g) {
+    lane_gradient_faces(a, sU, &sJmp[0][0][0], &sJmp[1][0][0], &sJmp[2][0][0], &sJmp[3][0][0], &sJmp[4][0][0],
+                        &sJmp[5][0][0], i, j, k, g);
 }
 
 template <bool PF, class UT, class CT>
@@ -760,24 +794,60 @@ struct alignas(4 * sizeof(C)) FvQuad {
 // dispatch_viscous_flux calls of compute_face_j / compute_face_j_wall,
 // rhs.cu:284-285, 318) from the Riemann-order state trace and the gradient
 // slice; only the 11 gradient components the direction reads are traced.
+// One face point: element e (state tile sUe, gradient tile sGe), face side
+// `side` of direction D, face point fp.
+template <int D, class UT, class CT>
+__device__ __forceinline__ void fv_point(const StageArgs<UT, CT>& a, const CT (*sUe)[64], const CT (*sGe)[64],
+                                         int64_t e, int side, int fp) {
+    int base, stride;
+    face_line(D, fp, base, stride);
+    const Coef4<CT> c = side_coef<CT>(side);
+    CT st[5], gt[15], fvv[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) st[v] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+#pragma unroll
+    for (int q = 0; q < 15; ++q) gt[q] = trace4_swz(sGe[q], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+    CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
+    if (a.mu_t) mu = mu + a.mu_t[e];
+    viscous_flux(D, st, gt, mu, fvv);
+    FvQuad<CT>* dst = reinterpret_cast<FvQuad<CT>*>(a.fv + e * kFvStride + (2 * D + side) * 64 + fp * 4);
+    *dst = FvQuad<CT>{{fvv[1], fvv[2], fvv[3], fvv[4]}};
+}
+
 template <int D, class UT, class CT>
 __device__ __forceinline__ void fv_pass(const StageArgs<UT, CT>& a, const CT (*sU)[5][64], const CT (*sG)[15][64],
                                         const Tile& tl, int lane) {
     const FaceTask t = face_task<D>(lane);
     if (!tile_valid(tl, t.le)) return;
-    int base, stride;
-    face_line(D, t.fp, base, stride);
-    const Coef4<CT> c = side_coef<CT>(t.side);
-    CT st[5], gt[15], fvv[5];
+    fv_point<D>(a, sU[t.le], sG[t.le], tile_eidx(tl, t.le), t.side, t.fp);
+}
+
+// Own-side face correction j from the interface flux fr of a face point
+// (compute_face_j / compute_face_j_wall, rhs.cu:273-339): own is the
+// element's trace, fvo / fvn the own and the neighbour's viscous normal flux
+// (component 0 zero), right = the face is the element's right face.
+template <int D, bool VISC, class CT>
+__device__ __forceinline__ void face_correction(const CT* fr, const CT* own, const CT* fvo, const CT* fvn, bool wall,
+                                                bool right, CT* jv) {
+    OD_CONTRACT_IN_EXPRESSIONS
+    CT fe[5];
+    euler_flux(D, own, fe);
+    CT wall_p = (CT)0;
+    if (D == 1) wall_p = fe[2] - dv(own[2] * own[2], own[0]);  // rhs.cu:323-326
 #pragma unroll
-    for (int v = 0; v < 5; ++v) st[v] = trace4(sU[t.le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
-#pragma unroll
-    for (int q = 0; q < 15; ++q) gt[q] = trace4_swz(sG[t.le][q], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
-    CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
-    if (a.mu_t) mu = mu + a.mu_t[tile_eidx(tl, t.le)];
-    viscous_flux(D, st, gt, mu, fvv);
-    FvQuad<CT>* dst = reinterpret_cast<FvQuad<CT>*>(a.fv + tile_eidx(tl, t.le) * kFvStride + t.f * 64 + t.fp * 4);
-    *dst = FvQuad<CT>{{fvv[1], fvv[2], fvv[3], fvv[4]}};
+    for (int v = 0; v < 5; ++v) {
+        const CT wf = (D == 1 && v == 2) ? wall_p : (CT)0.0;
+        if (VISC) {
+            const CT fl = right ? fvo[v] : fvn[v];
+            const CT frr = right ? fvn[v] : fvo[v];
+            CT fc = fr[v] - ((CT)0.5) * (fl + frr);
+            CT jn = fc - (fe[v] - fvo[v]);
+            CT jw = (wf - fvo[v]) - (fe[v] - fvo[v]);
+            jv[v] = wall ? jw : jn;
+        } else {
+            jv[v] = wall ? (wf - fe[v]) : (fr[v] - fe[v]);
+        }
+    }
 }
 
 // Interface flux at the element faces of direction D -> own-side correction
@@ -827,27 +897,11 @@ __device__ __forceinline__ void flux_pass(const StageArgs<UT, CT>& a, const CT (
         L[v] = right ? own[v] : oth[v];
         R[v] = right ? oth[v] : own[v];
     }
-    CT fr[5], fe[5];
+    CT fr[5], jv[5];
     interface_flux<D>(a.flux, L, R, fr);
-    euler_flux(D, own, fe);
-    CT wall_p = (CT)0;
-    if (D == 1) wall_p = fe[2] - dv(own[2] * own[2], own[0]);  // rhs.cu:323-326
+    face_correction<D, VISC>(fr, own, fvo, fvn, wall, right, jv);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        const CT wf = (D == 1 && v == 2) ? wall_p : (CT)0.0;
-        CT jv;
-        if (VISC) {
-            const CT fl = right ? fvo[v] : fvn[v];
-            const CT frr = right ? fvn[v] : fvo[v];
-            CT fc = fr[v] - ((CT)0.5) * (fl + frr);
-            CT jn = fc - (fe[v] - fvo[v]);
-            CT jw = (wf - fvo[v]) - (fe[v] - fvo[v]);
-            jv = wall ? jw : jn;
-        } else {
-            jv = wall ? (wf - fe[v]) : (fr[v] - fe[v]);
-        }
-        sJ[t.le][t.f][v][t.fp] = jv;
-    }
+    for (int v = 0; v < 5; ++v) sJ[t.le][t.f][v][t.fp] = jv[v];
 }
 
 // Index of this wave in its work-group, as a wave-uniform (scalar) value: the
@@ -909,13 +963,13 @@ __device__ __forceinline__ void load_state_tile(const UT* u, int64_t NP, int64_t
 // zero) -> the element's fv block after its face part.  S: this wave's
 // 256-float LDS staging area.  Used by the fast f32 build (kSplitVisc).
 template <class UT, class CT>
-__device__ __forceinline__ void visc_divergence(const StageArgs<UT, CT>& a, const Tile& tl, const CT (*sUe)[64],
-                                                const CT* g, CT* S, int wave, int i, int j, int k, int pt) {
+__device__ __forceinline__ void visc_divergence(const StageArgs<UT, CT>& a, int64_t e, const CT (*sUe)[64],
+                                                const CT* g, CT* S, int i, int j, int k, int pt) {
     CT st[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) st[v] = sUe[v][pt];
     CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
-    if (a.mu_t) mu = mu + a.mu_t[tile_eidx(tl, wave)];
+    if (a.mu_t) mu = mu + a.mu_t[e];
     const ViscPoint<CT> vp = viscous_point(st, g, mu, 7);
     CT acc[4] = {(CT)0, (CT)0, (CT)0, (CT)0};
 #pragma unroll
@@ -947,7 +1001,7 @@ __device__ __forceinline__ void visc_divergence(const StageArgs<UT, CT>& a, cons
             acc[q] += dd * jac;
         }
     }
-    FvQuad<CT>* dst = reinterpret_cast<FvQuad<CT>*>(a.fv + tile_eidx(tl, wave) * kFvStride + kFvFaces + pt * 4);
+    FvQuad<CT>* dst = reinterpret_cast<FvQuad<CT>*>(a.fv + e * kFvStride + kFvFaces + pt * 4);
     *dst = FvQuad<CT>{{acc[0], acc[1], acc[2], acc[3]}};
 }
 
@@ -1036,7 +1090,8 @@ __device__ __forceinline__ void visc_body(const StageArgs<UT, CT>& a, const Tile
     CT g[15];
     if (tile_valid(tl, wave)) lane_gradient(a, L.sU[wave], L.sJmp[wave], i, j, k, g);
     if constexpr (kSplitVisc<CT>) {
-        if (tile_valid(tl, wave)) visc_divergence(a, tl, L.sU[wave], g, L.pf + wave * 256, wave, i, j, k, pt);
+        if (tile_valid(tl, wave))
+            visc_divergence(a, tile_eidx(tl, wave), L.sU[wave], g, L.pf + wave * 256, i, j, k, pt);
     }
     if constexpr (PF) __syncthreads();  // sG overwrites the jumps the other wave may still read
     if (tile_valid(tl, wave)) {
@@ -1267,6 +1322,448 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWavesUp
     update_body<UT, CT, VISC>(a, make_tile(a), L, wave_index(), threadIdx.x & 63);
 }
 
+#if OD_FAST
+// ---------------------------------------------------------------------------
+// Fast f32 build, one box with even nx, ny, nz: the update kernel on blocks
+// of 2x2x2 elements (512 threads; wave w owns local element w = lx + 2 ly +
+// 4 lz in the point-wise phases).  Three of an element's six faces face
+// another element of the block.  Their neighbour traces come from the
+// block's LDS state tiles instead of HBM, and the interface flux of such an
+// internal face is evaluated once for both of its elements.  Every value is
+// formed from the same operands in the same order as in update_body, so each
+// element gets the same bits as in the two-element-tile kernels (which the
+// ensemble kernels and meshes with an odd dimension use).
+// ---------------------------------------------------------------------------
+constexpr int kBlk = 8;
+constexpr int kBlkPfChunks = kBlk * 3 * 20;  // external face traces, 16-byte chunks
+
+struct Block {
+    int32_t ex0, ey0, ez0;  // coordinates of local element 0 (even)
+    int32_t e0;             // its element index
+};
+
+template <class UT, class CT>
+__device__ __forceinline__ Block make_block(const StageArgs<UT, CT>& a) {
+    Block b;
+    b.ex0 = 2 * (int)blockIdx.x;
+    b.ey0 = 2 * (int)blockIdx.y;
+    b.ez0 = 2 * (int)blockIdx.z;
+    b.e0 = b.ex0 + a.nx * (b.ey0 + a.ny * b.ez0);
+    return b;
+}
+
+// Bit d of a local element index: its coordinate along d in the block, and
+// the side (0 left, 1 right) of its external face normal to d.
+__device__ __forceinline__ int blk_bit(int le, int d) { return (le >> d) & 1; }
+
+template <class UT, class CT>
+__device__ __forceinline__ int blk_elem(const StageArgs<UT, CT>& a, const Block& b, int le) {
+    return b.e0 + (le & 1) + a.nx * (((le >> 1) & 1) + a.ny * (le >> 2));
+}
+
+// Neighbour across the external face normal to d of local element le (as
+// make_elem forms it); -1 = slip wall.
+template <class UT, class CT>
+__device__ __forceinline__ int blk_ext_nb(const StageArgs<UT, CT>& a, const Block& b, int le, int d) {
+    const int e = blk_elem(a, b, le);
+    const int s = blk_bit(le, d);
+    if (d == 0) {
+        const int ex = b.ex0 + s;
+        return s ? e + (ex == a.nx - 1 ? 1 - a.nx : 1) : e + (ex == 0 ? a.nx - 1 : -1);
+    }
+    if (d == 1) {
+        const int ey = b.ey0 + s, sy = a.nx;
+        if (s) return (a.wall && ey == a.ny - 1) ? -1 : e + (ey == a.ny - 1 ? -(a.ny - 1) * sy : sy);
+        return (a.wall && ey == 0) ? -1 : e + (ey == 0 ? (a.ny - 1) * sy : -sy);
+    }
+    const int ez = b.ez0 + s, sz = a.nx * a.ny;
+    return s ? e + (ez == a.nz - 1 ? -(a.nz - 1) * sz : sz) : e + (ez == 0 ? (a.nz - 1) * sz : -sz);
+}
+
+// Traces of the element's three external faces (face 2d + bit d of le, one
+// face point per lane, lanes 48..63 idle), as write_traces_own forms them.
+__device__ __forceinline__ void blk_write_ext_traces(const float (*sUe)[64], float* dst, int le, int lane) {
+    const int d = lane >> 4, fp = lane & 15;
+    if (d > 2) return;
+    const int f = 2 * d + blk_bit(le, d);
+    int base, stride;
+    face_line(d, fp, base, stride);
+    const Coef4<float> c = side_coef<float>(f & 1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
+        dst[f * 80 + v * 16 + fp] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, trace_face_order(f));
+}
+
+// LDS of a block.  sJe[le][d]: the external face of direction d of element
+// le -- first the neighbour's facing trace (LDS-DMA, chunk id = 4 x its
+// float offset), then the element's correction j there; sJi[le][d]: j at
+// the internal face of direction d.
+struct BlockUpdateLds {
+    alignas(16) float sJe[kBlk][3][5][16];
+    float sJi[kBlk][3][5][16];
+    float sU[kBlk][5][64];
+    float sF[kBlk][5][64];  // volume-flux line staging, then the volume RHS
+};
+
+// External faces of direction D of the four elements 4h .. 4h+3: one face
+// point per lane.  The arithmetic is flux_pass's.
+template <int D, bool VISC>
+__device__ __forceinline__ void blk_ext_pass(const StageArgs<float, float>& a, const Block& b, BlockUpdateLds& L,
+                                             int h, int lane) {
+    const int le = 4 * h + (lane >> 4), fp = lane & 15;
+    const int side = blk_bit(le, D);
+    const bool right = side;
+    int base, stride;
+    face_line(D, fp, base, stride);
+    const Coef4<float> c = side_coef<float>(side);
+    float own[5], oth[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) own[v] = trace4(L.sU[le][v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+    const int nb = blk_ext_nb(a, b, le, D);
+    const bool wall = D == 1 && nb < 0;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const float far = L.sJe[le][D][v][fp];
+        oth[v] = wall ? own[v] : far;
+    }
+    float fvo[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, fvn[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (VISC) {
+        const int f = 2 * D + side;
+        const FvQuad<float> o =
+            *reinterpret_cast<const FvQuad<float>*>(a.fv + (int64_t)blk_elem(a, b, le) * kFvStride + f * 64 + fp * 4);
+        FvQuad<float> n = o;
+        if (!wall) n = *reinterpret_cast<const FvQuad<float>*>(a.fv + (int64_t)nb * kFvStride + (f ^ 1) * 64 + fp * 4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            fvo[1 + q] = o.c[q];
+            fvn[1 + q] = n.c[q];
+        }
+    }
+    float Lq[5], Rq[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        Lq[v] = right ? own[v] : oth[v];
+        Rq[v] = right ? oth[v] : own[v];
+    }
+    float fr[5], jv[5];
+    interface_flux<D>(a.flux, Lq, Rq, fr);
+    face_correction<D, VISC>(fr, own, fvo, fvn, wall, right, jv);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) L.sJe[le][D][v][fp] = jv[v];
+}
+
+// Internal faces of direction D: pair p = lane / 16 (left element lL, right
+// element lL + 2^D), one face point per lane; one interface flux for both
+// sides, each side's correction as flux_pass forms it.
+template <int D, bool VISC>
+__device__ __forceinline__ void blk_int_pass(const StageArgs<float, float>& a, const Block& b, BlockUpdateLds& L,
+                                             int lane) {
+    const int p = lane >> 4, fp = lane & 15;
+    const int lL = D == 0 ? 2 * p : (D == 1 ? ((p & 1) | ((p >> 1) << 2)) : p);
+    const int lR = lL + (1 << D);
+    int base, stride;
+    face_line(D, fp, base, stride);
+    const Coef4<float> cr = side_coef<float>(true), cl = side_coef<float>(false);
+    float tl[5], tr[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) tl[v] = trace4(L.sU[lL][v], base, stride, cr.c0, cr.c1, cr.c2, cr.c3, face_order(D));
+#pragma unroll
+    for (int v = 0; v < 5; ++v) tr[v] = trace4(L.sU[lR][v], base, stride, cl.c0, cl.c1, cl.c2, cl.c3, face_order(D));
+    float fvl[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, fvr[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    if (VISC) {
+        const FvQuad<float> ql = *reinterpret_cast<const FvQuad<float>*>(
+            a.fv + (int64_t)blk_elem(a, b, lL) * kFvStride + (2 * D + 1) * 64 + fp * 4);
+        const FvQuad<float> qr = *reinterpret_cast<const FvQuad<float>*>(
+            a.fv + (int64_t)blk_elem(a, b, lR) * kFvStride + (2 * D) * 64 + fp * 4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            fvl[1 + q] = ql.c[q];
+            fvr[1 + q] = qr.c[q];
+        }
+    }
+    float fr[5], jl[5], jr[5];
+    interface_flux<D>(a.flux, tl, tr, fr);
+    face_correction<D, VISC>(fr, tl, fvl, fvr, false, true, jl);
+    face_correction<D, VISC>(fr, tr, fvr, fvl, false, false, jr);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        L.sJi[lL][D][v][fp] = jl[v];
+        L.sJi[lR][D][v][fp] = jr[v];
+    }
+}
+
+// EXT: write the new state's traces of the external faces only (the block
+// visc kernel forms the internal ones from its LDS state tiles).
+template <bool VISC, bool EXT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void od_update_block_kernel(
+    StageArgs<float, float> a) {
+    __shared__ BlockUpdateLds L;
+    const int wave = wave_index(), pt = threadIdx.x & 63;
+    const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
+    const Block b = make_block(a);
+    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    const int64_t e = blk_elem(a, b, wave);
+    const int64_t gbase = e * 64 + pt;
+    const float cf = coef_load<float>((int)threadIdx.x);
+    {  // neighbour traces of the external faces: one 64-chunk LDS-DMA per wave
+        const int id = wave * 64 + pt;
+        if (id < kBlkPfChunks) {
+            const int le = id / 60, r = id - 60 * le, d = r / 20, c = r - 20 * d;
+            const int nb = blk_ext_nb(a, b, le, d);
+            const int src_e = nb >= 0 ? nb : blk_elem(a, b, le);
+            const int f = 2 * d + blk_bit(le, d);
+            const float* src = a.tr_in + (int64_t)src_e * kTraceStride + (f ^ 1) * 80 + c * 4;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)(&L.sJe[0][0][0][0] + wave * 256),
+                                             16, 0, 0);
+        }
+    }
+    load_state_tile(a.uin, NP, e, L.sU[wave], pt);
+    coef_store<float>((int)threadIdx.x, cf);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
+    __syncthreads();
+
+    // Volume RHS of the wave's element (update_body's fast f32 path).
+    float st[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) st[v] = L.sU[wave][v][pt];
+    float vdiv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (VISC) {
+        const FvQuad<float> src = *reinterpret_cast<const FvQuad<float>*>(a.fv + e * kFvStride + kFvFaces + pt * 4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vdiv[q] = src.c[q];
+    }
+    float rs[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float F[5];
+        euler_flux(d, st, F);
+        if (d == 0) {
+            const float* Dx = coef_lds<float>().D[i];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) rs[v] += quad_line(Dx, F[v]) * a.jx;
+            continue;
+        }
+        if (d > 1) wave_sync();  // the y lines have been read
+#pragma unroll
+        for (int v = 0; v < 5; ++v) L.sF[wave][v][pt] = F[v];
+        wave_sync();
+        const int idx = d == 1 ? j : k;
+        const int base = d == 1 ? i + 16 * k : i + 4 * j;
+        const int stride = d == 1 ? 4 : 16;
+        const float jac = d == 1 ? a.jy : a.jz;
+        const float* Dd = coef_lds<float>().D[idx];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            float dd = 0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) dd += Dd[m] * L.sF[wave][v][base + m * stride];
+            rs[v] += dd * jac;
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        float val = -rs[v];
+        if (VISC && v > 0) val += vdiv[v - 1];
+        if (v == 1) val += a.bf;
+        L.sF[wave][v][pt] = val;
+    }
+
+    // Interface fluxes: 6 external passes (4 elements x 16 points each) and
+    // 3 internal ones (4 pairs x 16 points), 9 wave passes over 8 waves.
+    switch (wave) {
+        case 0: blk_ext_pass<0, VISC>(a, b, L, 0, pt); blk_int_pass<2, VISC>(a, b, L, pt); break;
+        case 1: blk_ext_pass<0, VISC>(a, b, L, 1, pt); break;
+        case 2: blk_ext_pass<1, VISC>(a, b, L, 0, pt); break;
+        case 3: blk_ext_pass<1, VISC>(a, b, L, 1, pt); break;
+        case 4: blk_ext_pass<2, VISC>(a, b, L, 0, pt); break;
+        case 5: blk_ext_pass<2, VISC>(a, b, L, 1, pt); break;
+        case 6: blk_int_pass<0, VISC>(a, b, L, pt); break;
+        default: blk_int_pass<1, VISC>(a, b, L, pt); break;
+    }
+    __syncthreads();
+
+    // FR correction and stage update of the wave's element (rk.cu:12-25,
+    // 63-82; update_body's fast f32 path).
+    const CoefTable<float>& o = coef_lds<float>();
+    const float gLi = o.g[i][0] * a.jx, gRi = o.g[i][1] * a.jx;
+    const float gLj = o.g[j][0] * a.jy, gRj = o.g[j][1] * a.jy;
+    const float gLk = o.g[k][0] * a.jz, gRk = o.g[k][1] * a.jz;
+    const int fpx = j + 4 * k, fpy = i + 4 * k, fpz = i + 4 * j;
+    const int lx = wave & 1, ly = (wave >> 1) & 1, lz = wave >> 2;
+    const float* Jext = &L.sJe[wave][0][0][0];
+    const float* Jint = &L.sJi[wave][0][0][0];
+    const float* Jx0 = (lx == 0 ? Jext : Jint);
+    const float* Jx1 = (lx == 1 ? Jext : Jint);
+    const float* Jy0 = (ly == 0 ? Jext : Jint) + 80;
+    const float* Jy1 = (ly == 1 ? Jext : Jint) + 80;
+    const float* Jz0 = (lz == 0 ? Jext : Jint) + 160;
+    const float* Jz1 = (lz == 1 ? Jext : Jint) + 160;
+    float uprv[5], ucr[5], r[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        ucr[v] = L.sU[wave][v][pt];
+        r[v] = L.sF[wave][v][pt];
+    }
+    if (a.stage1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) uprv[v] = ucr[v];
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) uprv[v] = a.uprev[v * NP + gbase];
+    }
+    float unew[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        float cx = -(Jx0[v * 16 + fpx] * gLi + Jx1[v * 16 + fpx] * gRi);
+        float cy = -(Jy0[v * 16 + fpy] * gLj + Jy1[v * 16 + fpy] * gRj);
+        float cz = -(Jz0[v * 16 + fpz] * gLk + Jz1[v * 16 + fpz] * gRk);
+        float rt = r[v] + (cx + cy + cz);
+        if (a.mode == 1) {
+            a.dbg[v * NP + gbase] = rt;
+            unew[v] = ucr[v];
+            continue;
+        }
+        if (!a.accum_state) {
+            float val = a.stage1 ? (a.ca * uprv[v] + a.cc * rt) : (a.ca * uprv[v] + a.cb * ucr[v] + a.cc * rt);
+            unew[v] = val;
+        } else {
+            float rr = rt;
+            unew[v] = a.stage1 ? (a.ua * uprv[v] + a.uc * rr) : (a.ua * uprv[v] + a.ub * ucr[v] + a.uc * rr);
+        }
+        a.uout[v * NP + gbase] = unew[v];
+    }
+    if (a.mode == 1) return;
+
+    // Face traces of the new state.
+#pragma unroll
+    for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = unew[v];
+    wave_sync();
+    if constexpr (EXT) {
+        blk_write_ext_traces(L.sU[wave], a.tr_out + e * kTraceStride, wave, pt);
+    } else {
+        write_traces_own(L.sU[wave], a.tr_out + e * kTraceStride, pt, VISC);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The visc kernel on the same 2x2x2 blocks: the BR1 jumps of the internal
+// faces come from the block's LDS state tiles (one task per internal face
+// point forms both sides' jumps), those of the external faces from the
+// prefetched neighbour traces; gradient, volume viscous divergence and the
+// face viscous fluxes are visc_body's, element by element.  LDS: the jumps
+// [le][ext d0..d2, int d0..d2][5][16] (the external part first holds the
+// prefetched traces) share one buffer with the gradient tile [le][15][64]
+// (written after a barrier), and the state tiles.
+// ---------------------------------------------------------------------------
+struct BlockViscLds {
+    alignas(16) float buf[kBlk * 15 * 64];
+    float sU[kBlk][5][64];
+    __device__ float* jumps(int le) { return buf + le * 480; }
+    __device__ float (*sG())[15][64] { return reinterpret_cast<float(*)[15][64]>(buf); }
+};
+
+// BR1 jump at the external face of direction D of elements 4h .. 4h+3 (as br1_jumps).
+template <int D>
+__device__ __forceinline__ void blk_br1_ext(const StageArgs<float, float>& a, const Block& b, BlockViscLds& L, int h,
+                                            int lane) {
+    const int le = 4 * h + (lane >> 4), fp = lane & 15;
+    const int side = blk_bit(le, D);
+    int base, stride;
+    face_line(D, fp, base, stride);
+    const Coef4<float> c = side_coef<float>(side);
+    const bool wall = D == 1 && blk_ext_nb(a, b, le, D) < 0;
+    float* J = L.jumps(le) + D * 80;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        float own = trace4(L.sU[le][v], base, stride, c.c0, c.c1, c.c2, c.c3, br1_order(D));
+        const float sign = (v > 0 && v < 4) ? -1.0f : 1.0f;
+        const float far = J[v * 16 + fp];
+        float nbv = !wall ? far : own * sign;
+        J[v * 16 + fp] = 0.5f * (nbv + own) - own;
+    }
+}
+
+// BR1 jumps of both sides of the internal faces of direction D (pairs as in
+// blk_int_pass); each side's jump as br1_jumps forms it with the other
+// side's trace as the neighbour value.
+template <int D>
+__device__ __forceinline__ void blk_br1_int(BlockViscLds& L, int lane) {
+    const int p = lane >> 4, fp = lane & 15;
+    const int lL = D == 0 ? 2 * p : (D == 1 ? ((p & 1) | ((p >> 1) << 2)) : p);
+    const int lR = lL + (1 << D);
+    int base, stride;
+    face_line(D, fp, base, stride);
+    const Coef4<float> cr = side_coef<float>(true), cl = side_coef<float>(false);
+    float* JL = L.jumps(lL) + (3 + D) * 80;
+    float* JR = L.jumps(lR) + (3 + D) * 80;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const float tl = trace4(L.sU[lL][v], base, stride, cr.c0, cr.c1, cr.c2, cr.c3, br1_order(D));
+        const float tr = trace4(L.sU[lR][v], base, stride, cl.c0, cl.c1, cl.c2, cl.c3, br1_order(D));
+        JL[v * 16 + fp] = 0.5f * (tr + tl) - tl;
+        JR[v * 16 + fp] = 0.5f * (tl + tr) - tr;
+    }
+}
+
+__global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, float> a) {
+    __shared__ BlockViscLds L;
+    const int wave = wave_index(), pt = threadIdx.x & 63;
+    const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
+    const Block b = make_block(a);
+    const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
+    const int64_t e = blk_elem(a, b, wave);
+    const float cf = coef_load<float>((int)threadIdx.x);
+    if (pt < 60) {  // the element's three external neighbour traces: 60 16-byte LDS-DMA chunks
+        const int d = pt / 20, c = pt - 20 * d;
+        const int nb = blk_ext_nb(a, b, wave, d);
+        const int f = 2 * d + blk_bit(wave, d);
+        const float* src = a.tr_in + (nb >= 0 ? (int64_t)nb : e) * kTraceStride + (f ^ 1) * 80 + c * 4;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                         (__attribute__((address_space(3))) void*)L.jumps(wave), 16, 0, 0);
+    }
+    load_state_tile(a.uin, NP, e, L.sU[wave], pt);
+    coef_store<float>((int)threadIdx.x, cf);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    switch (wave) {
+        case 0: blk_br1_ext<0>(a, b, L, 0, pt); blk_br1_int<2>(L, pt); break;
+        case 1: blk_br1_ext<0>(a, b, L, 1, pt); break;
+        case 2: blk_br1_ext<1>(a, b, L, 0, pt); break;
+        case 3: blk_br1_ext<1>(a, b, L, 1, pt); break;
+        case 4: blk_br1_ext<2>(a, b, L, 0, pt); break;
+        case 5: blk_br1_ext<2>(a, b, L, 1, pt); break;
+        case 6: blk_br1_int<0>(L, pt); break;
+        default: blk_br1_int<1>(L, pt); break;
+    }
+    __syncthreads();
+    // the wave's element: face f = 2d + s has its jumps in the external slot
+    // d when s is the element's coordinate bit d, else in the internal slot
+    const float* Jw = L.jumps(wave);
+    const int lx = wave & 1, ly = (wave >> 1) & 1, lz = wave >> 2;
+    float g[15];
+    lane_gradient_faces(a, L.sU[wave], Jw + (lx == 0 ? 0 : 240), Jw + (lx == 1 ? 0 : 240), Jw + (ly == 0 ? 80 : 320),
+                        Jw + (ly == 1 ? 80 : 320), Jw + (lz == 0 ? 160 : 400), Jw + (lz == 1 ? 160 : 400), i, j, k, g);
+    // staging of the volume viscous-flux lines in the element's own jumps,
+    // which only this wave has read
+    visc_divergence(a, e, L.sU[wave], g, L.jumps(wave), i, j, k, pt);
+    __syncthreads();  // the gradient tile overwrites jumps other waves may still read
+#pragma unroll
+    for (int q = 0; q < 15; ++q) L.sG()[wave][q][swz64(pt)] = g[q];
+    __syncthreads();
+    // face viscous fluxes: pass (D, q) = elements 2q, 2q+1, both faces of direction D
+    const int le = 2 * (wave & 3) + (pt >> 5), side = (pt >> 4) & 1, fp = pt & 15;
+    const int64_t ef = blk_elem(a, b, le);
+    if (wave < 4) {
+        fv_point<0>(a, L.sU[le], L.sG()[le], ef, side, fp);
+        fv_point<2>(a, L.sU[le], L.sG()[le], ef, side, fp);
+    } else {
+        fv_point<1>(a, L.sU[le], L.sG()[le], ef, side, fp);
+    }
+}
+#endif
+
 // ---------------------------------------------------------------------------
 // Ensembles: the tiles of every box of a batch in one launch per phase.  A
 // work unit is `PAIRS` consecutive tiles (one 128-thread tile each); work-
@@ -1486,6 +1983,32 @@ int launch_update(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream) {
     hipLaunchKernelGGL((od_update_kernel<UT, CT, VISC>), grid_of(a), dim3(64 * kEpb), 0, (hipStream_t)stream, a);
     return check_launch();
 }
+
+#if OD_FAST
+inline dim3 block_grid(const StageArgs<float, float>& a) {
+    return dim3((unsigned)(a.nx / 2), (unsigned)(a.ny / 2), (unsigned)(a.nz / 2));
+}
+
+template <bool VISC>
+int launch_update_block(const StageArgs<float, float>& a, bool ext_traces, void* stream) {
+    if (a.nx % 2 || a.ny % 2 || a.nz % 2) return 1;  // blocks of 2x2x2 elements tile the box exactly
+    if (ext_traces)
+        hipLaunchKernelGGL((od_update_block_kernel<VISC, true>), block_grid(a), dim3(64 * kBlk), 0,
+                           (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL((od_update_block_kernel<VISC, false>), block_grid(a), dim3(64 * kBlk), 0,
+                           (hipStream_t)stream, a);
+    return check_launch();
+}
+template int launch_update_block<true>(const StageArgs<float, float>&, bool, void*);
+template int launch_update_block<false>(const StageArgs<float, float>&, bool, void*);
+
+int launch_visc_block(const StageArgs<float, float>& a, void* stream) {
+    if (a.nx % 2 || a.ny % 2 || a.nz % 2) return 1;
+    hipLaunchKernelGGL(od_visc_block_kernel, block_grid(a), dim3(64 * kBlk), 0, (hipStream_t)stream, a);
+    return check_launch();
+}
+#endif
 
 template <class UT, class CT>
 int launch_batch_visc(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int pairs, int groups, void* stream) {

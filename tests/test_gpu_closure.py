@@ -329,3 +329,19 @@ def test_closure_gains_over_the_plain_coarse_run():
     assert clos["errs"][0] is not None and plain["errs"][0] is not None
     assert clos["errs"][0] < 0.9 * plain["errs"][0]
     assert plain["errs"][1] is None and clos["errs"][1] is not None
+
+
+def test_constant_coefficient_baseline_is_constant():
+    """The constant-coefficient baseline (closure.constant_model) gives the
+    same C = s for every element, whatever its features."""
+    from opendynamo_amd import closure as C
+    announce(test="constant_model")
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(4096, 12)).astype(np.float32)
+    shift, scale = np.zeros(12, np.float32), np.ones(12, np.float32)
+    lo, hi = np.full(12, -1e9, np.float32), np.full(12, 1e9, np.float32)
+    s = 0.0123
+    with C.constant_model(s, shift, scale, lo, hi) as m:
+        y = m.forward(x)
+    assert np.all(y == y[0])
+    assert abs(y[0] - s) <= 5e-3 * s  # softplus(b4) = 1 up to the BF16 rounding of b4
