@@ -112,6 +112,26 @@ od_status od_rhs(od_ctx* ctx, void* host_out, size_t n_values);
  * fixed-order FP64 reduction on the device). */
 od_status od_conservation_sums(od_ctx* ctx, double out[5]);
 
+/* Allocation-free variant of od_conservation_sums.  No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under hipGraph stream capture, serving loops and harnesses that forbid
+ * hipMalloc) and hold the state in their own device memory.
+ *   d_state  device pointer, var-major [5][nx*ny*nz*64] of type u_real
+ *   d_out    device pointer to 5 doubles: the GL integrals (same kernels,
+ *            same fixed reduction order: bitwise equal to
+ *            od_conservation_sums on the same state and extents)
+ *   stream   a hipStream_t (NULL: the null stream); the call only enqueues
+ *            two kernels on it and returns, the caller synchronizes.
+ * The per-element partial sums live in a fixed device array of the library
+ * (5 * OD_NOALLOC_MAX_ELEMENTS doubles = 40 MiB on each GPU the library is
+ * loaded on, held until the process ends); meshes above
+ * OD_NOALLOC_MAX_ELEMENTS elements are refused with OD_ERR_INVALID.  The call
+ * makes no hipMalloc / hipFree.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (different streams, or threads) race. */
+#define OD_NOALLOC_MAX_ELEMENTS 1048576
+od_status od_conservation_sums_noalloc(int32_t nx, int32_t ny, int32_t nz, double lx, double ly, double lz,
+                                       const void* d_state, od_real u_real, double* d_out, void* stream);
+
 /* CFL time step from the current state:
  *   dt = cfl / ((2P+1) * max_points sum_d (|u_d| + c) / dx_d), rounded to f32. */
 od_status od_cfl_dt(od_ctx* ctx, double cfl, double* dt_out);

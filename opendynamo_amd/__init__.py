@@ -29,8 +29,8 @@ def _state_dtype(default, dtype):
 EXPORTS = [
     "od_default_params", "od_create", "od_destroy", "od_set_state", "od_get_state",
     "od_get_state_async", "od_wait", "od_release", "od_step", "od_synchronize", "od_rhs",
-    "od_conservation_sums", "od_cfl_dt", "od_time_steps", "od_time_kernels", "od_info",
-    "od_last_error",
+    "od_conservation_sums", "od_conservation_sums_noalloc", "od_cfl_dt", "od_time_steps", "od_time_kernels",
+    "od_info", "od_last_error",
     "od_batch_create", "od_batch_destroy", "od_batch_set_state", "od_batch_get_state",
     "od_batch_step", "od_batch_synchronize", "od_batch_cfl_dt", "od_batch_conservation_sums",
     "od_batch_set_schedule", "od_batch_get_schedule", "od_batch_autotune", "od_batch_time_steps",
@@ -112,6 +112,8 @@ def lib():
         L.od_synchronize.argtypes = [vp]
         L.od_rhs.argtypes = [vp, vp, ctypes.c_size_t]
         L.od_conservation_sums.argtypes = [vp, dp]
+        L.od_conservation_sums_noalloc.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_double, vp, ctypes.c_int, vp, vp]
         L.od_cfl_dt.argtypes = [vp, ctypes.c_double, dp]
         L.od_time_steps.argtypes = [vp, ctypes.c_double, ctypes.c_uint32, dp]
         L.od_time_kernels.argtypes = [vp, ctypes.c_double, ctypes.c_uint32, dp,

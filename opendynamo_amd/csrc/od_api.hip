@@ -634,3 +634,72 @@ od_status od_info(od_ctx* c, double out[4]) {
 }
 
 }  // extern "C"
+
+// ---- Conservation sums without allocation -----------------------------------
+// od_conservation_sums_noalloc keeps its per-element partial sums ([5][E]
+// doubles, the buffer od_create allocates for od_conservation_sums) in this
+// fixed array.  Host code never uses the name `scratch` as an address: each
+// GPU holds its own copy, whose device address hipGetSymbolAddress returns
+// once per GPU (scratch_on below).
+static __device__ __align__(256) double scratch[5 * OD_NOALLOC_MAX_ELEMENTS];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up on the first call
+// on that GPU (with that GPU current) and kept.
+od_status scratch_on(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(OD_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(OD_ERR_HIP, "hipGetSymbolAddress returned a null address for the scratch array");
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return OD_OK;
+}
+
+// GL weights of the 64 points (tensor products of the 4-point rule), as
+// od_conservation_sums forms them.
+const double* gl_weights64() {
+    static const std::vector<double> w3 = [] {
+        const HostOps o = fr_operators();
+        std::vector<double> w(64);
+        for (int pt = 0; pt < 64; ++pt) w[pt] = o.w[pt & 3] * o.w[(pt >> 2) & 3] * o.w[pt >> 4];
+        return w;
+    }();
+    return w3.data();
+}
+
+}  // namespace
+
+extern "C" od_status od_conservation_sums_noalloc(int32_t nx, int32_t ny, int32_t nz, double lx, double ly,
+                                                  double lz, const void* d_state, od_real u_real, double* d_out,
+                                                  void* stream) {
+    if (!d_state || !d_out) return fail(OD_ERR_INVALID, "od_conservation_sums_noalloc: null argument");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(OD_ERR_INVALID, "mesh dimensions must be >= 1");
+    if (!real_ok(u_real)) return fail(OD_ERR_INVALID, "u_real must be OD_F32 or OD_F64");
+    if (!(lx > 0 && ly > 0 && lz > 0)) return fail(OD_ERR_INVALID, "domain extents must be positive");
+    const int64_t E = (int64_t)nx * ny * nz;
+    if (E > (int64_t)OD_NOALLOC_MAX_ELEMENTS)
+        return fail(OD_ERR_INVALID, "od_conservation_sums_noalloc: " + std::to_string(E) +
+                                        " elements exceed the maximum of " +
+                                        std::to_string((long long)OD_NOALLOC_MAX_ELEMENTS));
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    double* partial = nullptr;
+    od_status s = scratch_on(device, &partial);
+    if (s != OD_OK) return s;
+    const double jac = (lx / nx) * (ly / ny) * (lz / nz) / 8.0;
+    const int rc = u_real == OD_F32 ? od::launch_sums<float>((const float*)d_state, partial, d_out, E, gl_weights64(),
+                                                             jac, stream)
+                                    : od::launch_sums<double>((const double*)d_state, partial, d_out, E,
+                                                              gl_weights64(), jac, stream);
+    if (rc) return fail(OD_ERR_HIP, "conservation sum kernels failed to launch");
+    return OD_OK;
+}
