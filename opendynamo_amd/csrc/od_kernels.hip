@@ -201,6 +201,14 @@ __device__ __forceinline__ void viscous_flux(int d, const C* s, const C* g, C mu
 template <class C>
 __device__ __forceinline__ C sound_speed(C p, C rho) {
     OD_CONTRACT_IN_EXPRESSIONS
+#if OD_FAST
+    if constexpr (sizeof(C) == 4) {
+        // 1 / max(1e-7, rho) as a select: the reciprocal of rho is the one
+        // the fluxes form anyway (one transcendental fewer per state)
+        const C inv = rho >= (C)1e-7 ? __builtin_amdgcn_rcpf(rho) : __builtin_amdgcn_rcpf((C)1e-7);
+        return __builtin_amdgcn_sqrtf(fmax(((C)0.0), (((C)1.4) * p) * inv));
+    }
+#endif
     return sq(fmax(((C)0.0), dv(((C)1.4) * p, fmax(((C)1e-7), rho))));
 }
 
@@ -479,6 +487,11 @@ __device__ __forceinline__ constexpr int br1_order(int d) { return (d == 2 && !k
 // Order of the Riemann inputs and gradient slices (rhs.cu:126-180, 540-543,
 // 663-674, 716-727): x sequential from the first product, y pairwise, z from 0.
 __device__ __forceinline__ constexpr int face_order(int d) { return kOneX ? 0 : (d == 0 ? 2 : (d == 1 ? 0 : 1)); }
+// Order of the state and gradient traces the viscous face fluxes take: the
+// reference's face order; the fast build, whose viscous traces are formed and
+// used inside one kernel, takes the fused sequential form (one instruction
+// fewer per trace than the pairwise one).
+__device__ __forceinline__ constexpr int fv_order(int d) { return kOneX ? 1 : face_order(d); }
 // Faces of the trace layout the build writes, and the direction and order of face f.
 constexpr int kOwnFaces = kOneX ? 6 : 8;
 __device__ __forceinline__ int trace_face_dir(int f) { return f < 6 ? f >> 1 : 0; }
@@ -804,9 +817,9 @@ __device__ __forceinline__ void fv_point(const StageArgs<UT, CT>& a, const CT (*
     const Coef4<CT> c = side_coef<CT>(side);
     CT st[5], gt[15], fvv[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) st[v] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+    for (int v = 0; v < 5; ++v) st[v] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, fv_order(D));
 #pragma unroll
-    for (int q = 0; q < 15; ++q) gt[q] = trace4_swz(sGe[q], base, stride, c.c0, c.c1, c.c2, c.c3, face_order(D));
+    for (int q = 0; q < 15; ++q) gt[q] = trace4_swz(sGe[q], base, stride, c.c0, c.c1, c.c2, c.c3, fv_order(D));
     CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
     if (a.mu_t) mu = mu + a.mu_t[e];
     viscous_flux(D, st, gt, mu, fvv);
@@ -1394,6 +1407,14 @@ __device__ __forceinline__ void blk_write_ext_traces(const float (*sUe)[64], flo
         dst[f * 80 + v * 16 + fp] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, trace_face_order(f));
 }
 
+// State tiles of a block's elements, one every 5 * 64 + 16 floats: the pad
+// puts consecutive elements' tiles 16 banks apart, so the two elements whose
+// face points one half-wave reads in a face pass do not share banks.
+struct BlockTiles {
+    float t[kBlk][5 * 64 + 16];
+    __device__ float (*operator[](int le))[64] { return reinterpret_cast<float(*)[64]>(t[le]); }
+};
+
 // LDS of a block.  sJe[le][d]: the external face of direction d of element
 // le -- first the neighbour's facing trace (LDS-DMA, chunk id = 4 x its
 // float offset), then the element's correction j there; sJi[le][d]: j at
@@ -1401,7 +1422,7 @@ __device__ __forceinline__ void blk_write_ext_traces(const float (*sUe)[64], flo
 struct BlockUpdateLds {
     alignas(16) float sJe[kBlk][3][5][16];
     float sJi[kBlk][3][5][16];
-    float sU[kBlk][5][64];
+    BlockTiles sU;
     float sF[kBlk][5][64];  // volume-flux line staging, then the volume RHS
 };
 
@@ -1659,8 +1680,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void o
 // ---------------------------------------------------------------------------
 struct BlockViscLds {
     alignas(16) float buf[kBlk * 15 * 64];
-    float sU[kBlk][5][64];
-    __device__ float* jumps(int le) { return buf + le * 480; }
+    BlockTiles sU;
+    // 480 jumps per element, 496 apart (the tiles' bank offset again)
+    __device__ float* jumps(int le) { return buf + le * 496; }
     __device__ float (*sG())[15][64] { return reinterpret_cast<float(*)[15][64]>(buf); }
 };
 
