@@ -13,7 +13,11 @@
 //                                 element's own faces xL,xR,yL,yR,zL,zR, then
 //                                 [64][4] the volume viscous-flux divergence of
 //                                 components 1..4 at the solution points (fast
-//                                 f32 build only); component fastest
+//                                 f32 build only); component fastest.  The
+//                                 fast f32 kernels on 2x2x2-element blocks
+//                                 store, at each face interior to a block,
+//                                 the half-difference (F_vL - F_vR) / 2 in
+//                                 the left element's slot only
 #pragma once
 #include <cstddef>
 #include <cstdint>

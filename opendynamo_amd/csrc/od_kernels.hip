@@ -482,6 +482,12 @@ __device__ __forceinline__ CT quad_line(const CT* Dd, CT f) {
     dd += Dd[3] * b3;
     return dd;
 }
+// Fast f32 build: the interface correction takes the viscous fluxes of the
+// two sides as their half-difference (F_vL - F_vR) / 2, which the block visc
+// kernel stores once per interior face instead of the two fluxes; the strict
+// build keeps the reference's F* = F_riemann - (F_vL + F_vR) / 2.
+template <class CT>
+constexpr bool kHalfDiff = OD_FAST && std::is_same<CT, float>::value;
 // Order of the BR1 traces (rhs.cu:385-455): x, y pairwise; z sequential from 0.
 __device__ __forceinline__ constexpr int br1_order(int d) { return (d == 2 && !kOneX) ? 1 : 0; }
 // Order of the Riemann inputs and gradient slices (rhs.cu:126-180, 540-543,
@@ -810,8 +816,8 @@ struct alignas(4 * sizeof(C)) FvQuad {
 // One face point: element e (state tile sUe, gradient tile sGe), face side
 // `side` of direction D, face point fp.
 template <int D, class UT, class CT>
-__device__ __forceinline__ void fv_point(const StageArgs<UT, CT>& a, const CT (*sUe)[64], const CT (*sGe)[64],
-                                         int64_t e, int side, int fp) {
+__device__ __forceinline__ FvQuad<CT> fv_value(const StageArgs<UT, CT>& a, const CT (*sUe)[64],
+                                               const CT (*sGe)[64], int64_t e, int side, int fp) {
     int base, stride;
     face_line(D, fp, base, stride);
     const Coef4<CT> c = side_coef<CT>(side);
@@ -823,8 +829,14 @@ __device__ __forceinline__ void fv_point(const StageArgs<UT, CT>& a, const CT (*
     CT mu = a.sutherland ? sutherland(st, a.mu_ref, a.t_ref, a.s_suth) : a.mu;
     if (a.mu_t) mu = mu + a.mu_t[e];
     viscous_flux(D, st, gt, mu, fvv);
-    FvQuad<CT>* dst = reinterpret_cast<FvQuad<CT>*>(a.fv + e * kFvStride + (2 * D + side) * 64 + fp * 4);
-    *dst = FvQuad<CT>{{fvv[1], fvv[2], fvv[3], fvv[4]}};
+    return FvQuad<CT>{{fvv[1], fvv[2], fvv[3], fvv[4]}};
+}
+
+template <int D, class UT, class CT>
+__device__ __forceinline__ void fv_point(const StageArgs<UT, CT>& a, const CT (*sUe)[64], const CT (*sGe)[64],
+                                         int64_t e, int side, int fp) {
+    *reinterpret_cast<FvQuad<CT>*>(a.fv + e * kFvStride + (2 * D + side) * 64 + fp * 4) =
+        fv_value<D>(a, sUe, sGe, e, side, fp);
 }
 
 template <int D, class UT, class CT>
@@ -853,14 +865,32 @@ __device__ __forceinline__ void face_correction(const CT* fr, const CT* own, con
         if (VISC) {
             const CT fl = right ? fvo[v] : fvn[v];
             const CT frr = right ? fvn[v] : fvo[v];
-            CT fc = fr[v] - ((CT)0.5) * (fl + frr);
-            CT jn = fc - (fe[v] - fvo[v]);
+            CT jn;
+            if constexpr (kHalfDiff<CT>) {
+                // F* - (F_e - F_v,own) = (F_riemann - F_e) +- (F_vL - F_vR) / 2
+                const CT hd = ((CT)0.5) * (fl - frr);
+                jn = (fr[v] - fe[v]) + (right ? hd : -hd);
+            } else {
+                CT fc = fr[v] - ((CT)0.5) * (fl + frr);
+                jn = fc - (fe[v] - fvo[v]);
+            }
             CT jw = (wf - fvo[v]) - (fe[v] - fvo[v]);
             jv[v] = wall ? jw : jn;
         } else {
             jv[v] = wall ? (wf - fe[v]) : (fr[v] - fe[v]);
         }
     }
+}
+
+// The same correction at an interior face whose viscous half-difference
+// hd = (F_vL - F_vR) / 2 (component 0 zero) is stored (fast f32 build).
+template <int D, class CT>
+__device__ __forceinline__ void face_correction_hd(const CT* fr, const CT* own, const CT* hd, bool right, CT* jv) {
+    OD_CONTRACT_IN_EXPRESSIONS
+    CT fe[5];
+    euler_flux(D, own, fe);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) jv[v] = (fr[v] - fe[v]) + (right ? hd[v] : -hd[v]);
 }
 
 // Interface flux at the element faces of direction D -> own-side correction
@@ -1490,22 +1520,21 @@ __device__ __forceinline__ void blk_int_pass(const StageArgs<float, float>& a, c
     for (int v = 0; v < 5; ++v) tl[v] = trace4(L.sU[lL][v], base, stride, cr.c0, cr.c1, cr.c2, cr.c3, face_order(D));
 #pragma unroll
     for (int v = 0; v < 5; ++v) tr[v] = trace4(L.sU[lR][v], base, stride, cl.c0, cl.c1, cl.c2, cl.c3, face_order(D));
-    float fvl[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, fvr[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    if (VISC) {
-        const FvQuad<float> ql = *reinterpret_cast<const FvQuad<float>*>(
-            a.fv + (int64_t)blk_elem(a, b, lL) * kFvStride + (2 * D + 1) * 64 + fp * 4);
-        const FvQuad<float> qr = *reinterpret_cast<const FvQuad<float>*>(
-            a.fv + (int64_t)blk_elem(a, b, lR) * kFvStride + (2 * D) * 64 + fp * 4);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            fvl[1 + q] = ql.c[q];
-            fvr[1 + q] = qr.c[q];
-        }
-    }
     float fr[5], jl[5], jr[5];
     interface_flux<D>(a.flux, tl, tr, fr);
-    face_correction<D, VISC>(fr, tl, fvl, fvr, false, true, jl);
-    face_correction<D, VISC>(fr, tr, fvr, fvl, false, false, jr);
+    if constexpr (VISC) {
+        // the half-difference the block visc kernel stored in the left
+        // element's slot of the face; component 0 as the tile kernels form it
+        const FvQuad<float> q = *reinterpret_cast<const FvQuad<float>*>(
+            a.fv + (int64_t)blk_elem(a, b, lL) * kFvStride + (2 * D + 1) * 64 + fp * 4);
+        const float hd[5] = {0.5f * (0.f - 0.f), q.c[0], q.c[1], q.c[2], q.c[3]};
+        face_correction_hd<D>(fr, tl, hd, true, jl);
+        face_correction_hd<D>(fr, tr, hd, false, jr);
+    } else {
+        const float zero[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+        face_correction<D, VISC>(fr, tl, zero, zero, false, true, jl);
+        face_correction<D, VISC>(fr, tr, zero, zero, false, false, jr);
+    }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         L.sJi[lL][D][v][fp] = jl[v];
@@ -1729,6 +1758,35 @@ __device__ __forceinline__ void blk_br1_int(BlockViscLds& L, int lane) {
     }
 }
 
+// Viscous normal fluxes at the faces of direction D of the q-th element pair
+// along D (left lL, right lL + 2^D): lane = (element of the pair, side,
+// face point).  External faces store their flux as the tile kernels do; at
+// the pair's interior face the left element's lane (16..31) takes the right
+// element's flux from lane + 16 (32..47) and stores the half-difference
+// (F_vL - F_vR) / 2 in the left element's slot, which is all the update
+// kernel reads there.
+template <int D>
+__device__ __forceinline__ void blk_fv_pass(const StageArgs<float, float>& a, const Block& b, BlockViscLds& L, int q,
+                                            int pt) {
+    const int lL = D == 0 ? 2 * q : (D == 1 ? ((q & 1) | ((q >> 1) << 2)) : q);
+    const int second = pt >> 5, side = (pt >> 4) & 1, fp = pt & 15;
+    const int le = second ? lL + (1 << D) : lL;
+    const int64_t e = blk_elem(a, b, le);
+    const FvQuad<float> f = fv_value<D>(a, L.sU[le], L.sG()[le], e, side, fp);
+    FvQuad<float> g;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) g.c[c] = __shfl_xor(f.c[c], 48);  // lane 16 + fp <-> 32 + fp
+    FvQuad<float>* dst = reinterpret_cast<FvQuad<float>*>(a.fv + e * kFvStride + (2 * D + side) * 64 + fp * 4);
+    if (second == side) {
+        *dst = f;  // external face
+    } else if (!second) {
+        FvQuad<float> h;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) h.c[c] = 0.5f * (f.c[c] - g.c[c]);
+        *dst = h;
+    }
+}
+
 __global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, float> a) {
     __shared__ BlockViscLds L;
     const int wave = wave_index(), pt = threadIdx.x & 63;
@@ -1774,14 +1832,13 @@ __global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, flo
 #pragma unroll
     for (int q = 0; q < 15; ++q) L.sG()[wave][q][swz64(pt)] = g[q];
     __syncthreads();
-    // face viscous fluxes: pass (D, q) = elements 2q, 2q+1, both faces of direction D
-    const int le = 2 * (wave & 3) + (pt >> 5), side = (pt >> 4) & 1, fp = pt & 15;
-    const int64_t ef = blk_elem(a, b, le);
+    // face viscous fluxes: pass (D, q) = the q-th pair of elements along D,
+    // both faces of direction D (waves 0..3: x then z, waves 4..7: y)
     if (wave < 4) {
-        fv_point<0>(a, L.sU[le], L.sG()[le], ef, side, fp);
-        fv_point<2>(a, L.sU[le], L.sG()[le], ef, side, fp);
+        blk_fv_pass<0>(a, b, L, wave, pt);
+        blk_fv_pass<2>(a, b, L, wave, pt);
     } else {
-        fv_point<1>(a, L.sU[le], L.sG()[le], ef, side, fp);
+        blk_fv_pass<1>(a, b, L, wave - 4, pt);
     }
 }
 #endif
