@@ -1378,7 +1378,6 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWavesUp
 // ensemble kernels and meshes with an odd dimension use).
 // ---------------------------------------------------------------------------
 constexpr int kBlk = 8;
-constexpr int kBlkPfChunks = kBlk * 3 * 20;  // external face traces, 16-byte chunks
 
 struct Block {
     int32_t ex0, ey0, ez0;  // coordinates of local element 0 (even)
@@ -1405,22 +1404,37 @@ __device__ __forceinline__ int blk_elem(const StageArgs<UT, CT>& a, const Block&
 }
 
 // Neighbour across the external face normal to d of local element le (as
-// make_elem forms it); -1 = slip wall.
+// make_elem forms it); -1 = slip wall.  Branch-free: the offsets of the
+// block's left (bit d = 0) and right (bit d = 1) external neighbours are
+// block-uniform, each lane selects by its element's bit.
 template <class UT, class CT>
 __device__ __forceinline__ int blk_ext_nb(const StageArgs<UT, CT>& a, const Block& b, int le, int d) {
     const int e = blk_elem(a, b, le);
     const int s = blk_bit(le, d);
-    if (d == 0) {
-        const int ex = b.ex0 + s;
-        return s ? e + (ex == a.nx - 1 ? 1 - a.nx : 1) : e + (ex == 0 ? a.nx - 1 : -1);
-    }
-    if (d == 1) {
-        const int ey = b.ey0 + s, sy = a.nx;
-        if (s) return (a.wall && ey == a.ny - 1) ? -1 : e + (ey == a.ny - 1 ? -(a.ny - 1) * sy : sy);
-        return (a.wall && ey == 0) ? -1 : e + (ey == 0 ? (a.ny - 1) * sy : -sy);
-    }
-    const int ez = b.ez0 + s, sz = a.nx * a.ny;
-    return s ? e + (ez == a.nz - 1 ? -(a.nz - 1) * sz : sz) : e + (ez == 0 ? (a.nz - 1) * sz : -sz);
+    const int n = d == 0 ? a.nx : (d == 1 ? a.ny : a.nz);
+    const int c0 = d == 0 ? b.ex0 : (d == 1 ? b.ey0 : b.ez0);  // the block's left coordinate along d
+    const int stride = d == 0 ? 1 : (d == 1 ? a.nx : a.nx * a.ny);
+    const int off_l = (c0 == 0 ? n - 1 : -1) * stride;
+    const int off_r = (c0 + 1 == n - 1 ? 1 - n : 1) * stride;
+    const bool wall_l = d == 1 && a.wall && c0 == 0;
+    const bool wall_r = d == 1 && a.wall && c0 + 1 == n - 1;
+    return (s ? wall_r : wall_l) ? -1 : e + (s ? off_r : off_l);
+}
+
+// Source of 16-byte chunk `chunk` (0..59) of the external-face traces of
+// local element le, as the LDS prefetch lays them out ([d][5][16]): the
+// neighbour's facing trace (its own at a slip wall, where it is not used).
+// le is wave-uniform, so the three neighbours are scalar work and each lane
+// only selects its direction's.
+template <class UT, class CT>
+__device__ __forceinline__ const UT* blk_ext_trace_src(const StageArgs<UT, CT>& a, const Block& b, int le,
+                                                       int chunk) {
+    const int nb0 = blk_ext_nb(a, b, le, 0), nb1 = blk_ext_nb(a, b, le, 1), nb2 = blk_ext_nb(a, b, le, 2);
+    const int d = chunk >= 40 ? 2 : (chunk >= 20 ? 1 : 0), c = chunk - 20 * d;
+    const int nb = d == 0 ? nb0 : (d == 1 ? nb1 : nb2);
+    const int src_e = nb >= 0 ? nb : blk_elem(a, b, le);
+    const int f = 2 * d + blk_bit(le, d);
+    return a.tr_in + (int64_t)src_e * kTraceStride + (f ^ 1) * 80 + c * 4;
 }
 
 // Traces of the element's three external faces (face 2d + bit d of le, one
@@ -1555,18 +1569,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void o
     const int64_t e = blk_elem(a, b, wave);
     const int64_t gbase = e * 64 + pt;
     const float cf = coef_load<float>((int)threadIdx.x);
-    {  // neighbour traces of the external faces: one 64-chunk LDS-DMA per wave
-        const int id = wave * 64 + pt;
-        if (id < kBlkPfChunks) {
-            const int le = id / 60, r = id - 60 * le, d = r / 20, c = r - 20 * d;
-            const int nb = blk_ext_nb(a, b, le, d);
-            const int src_e = nb >= 0 ? nb : blk_elem(a, b, le);
-            const int f = 2 * d + blk_bit(le, d);
-            const float* src = a.tr_in + (int64_t)src_e * kTraceStride + (f ^ 1) * 80 + c * 4;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)(&L.sJe[0][0][0][0] + wave * 256),
-                                             16, 0, 0);
-        }
+    if (pt < 60) {  // the element's three external neighbour traces: 60 16-byte LDS-DMA chunks
+        const float* src = blk_ext_trace_src(a, b, wave, pt);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                         (__attribute__((address_space(3))) void*)&L.sJe[wave][0][0][0], 16, 0, 0);
     }
     load_state_tile(a.uin, NP, e, L.sU[wave], pt);
     coef_store<float>((int)threadIdx.x, cf);
@@ -1796,10 +1802,7 @@ __global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, flo
     const int64_t e = blk_elem(a, b, wave);
     const float cf = coef_load<float>((int)threadIdx.x);
     if (pt < 60) {  // the element's three external neighbour traces: 60 16-byte LDS-DMA chunks
-        const int d = pt / 20, c = pt - 20 * d;
-        const int nb = blk_ext_nb(a, b, wave, d);
-        const int f = 2 * d + blk_bit(wave, d);
-        const float* src = a.tr_in + (nb >= 0 ? (int64_t)nb : e) * kTraceStride + (f ^ 1) * 80 + c * 4;
+        const float* src = blk_ext_trace_src(a, b, wave, pt);
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                          (__attribute__((address_space(3))) void*)L.jumps(wave), 16, 0, 0);
     }
