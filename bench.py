@@ -319,9 +319,10 @@ def main():
         "bound": "hbm", "kernel": upd_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
         # the roofline is priced on HBM bytes; what limits the kernels as
-        # measured is VALU issue and HBM together (rocprofv3 SQ_INSTS_VALU
-        # and FETCH/WRITE_SIZE in the committed profile)
-        "limiter": "valu+hbm (measured)",
+        # measured (rocprofv3 FETCH/WRITE_SIZE and SQ_INSTS_VALU in the
+        # committed profile): the update kernel moves its bytes at the rate
+        # of a plain device copy, the visc kernel is VALU-issue bound
+        "limiter": "hbm (update kernel, at the device-copy rate) + valu (visc kernel)",
         "traffic_source": f"{PMC_PROFILE}: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of {upd_name} in separate "
                           "profiling runs of this workload, copy-calibrated (not measured in this run)",
         "algorithmic_bytes_per_launch": alg_bytes_launch, "launch_ms": round(upd_ms, 5),
