@@ -157,6 +157,17 @@ def initial_state(n, L, seed, n_modes, amplitude, kmax):
                             kmax=kmax, dtype=np.float32)
 
 
+# progress lines of long runs (set by main): at most one per PROGRESS_S seconds
+PROGRESS = {"log": None, "last": 0.0}
+PROGRESS_S = 30.0
+
+
+def _progress(msg):
+    if PROGRESS["log"] is not None and time.perf_counter() - PROGRESS["last"] >= PROGRESS_S:
+        PROGRESS["log"](msg)
+        PROGRESS["last"] = time.perf_counter()
+
+
 def run(n, L, mu, seed, cfg, times, mlp=None, update_interval=5, budget_kappa=1.0, cfl=0.3):
     """Advance from the IC, returning the state at each of `times` (ascending)
     and the GPU wall time of the stepping.  dt from the CFL of the IC."""
@@ -179,6 +190,7 @@ def run(n, L, mu, seed, cfg, times, mlp=None, update_interval=5, budget_kappa=1.
             t += n_steps * dt
             step += n_steps
             out.append(s.get_state())
+            _progress(f"  {n}^3 seed {seed}: t = {t:.2f} ({step} steps, {wall:.1f} s)")
         stats = s.closure_stats() if mlp is not None else None
     return dict(states=out, wall=wall, dt=dt, steps=step, stats=stats)
 
@@ -266,6 +278,10 @@ def build_dataset(cfg, log=print):
             res = run(nf, cfg["L"], mu, seed, cfg, cfg["snap_times"])
             finite = [bool(np.isfinite(st).all()) for st in res["states"]]
             log(f"fine seed {seed}: {res['steps']} steps, {res['wall']:.1f} s, finite snapshots {finite}")
+            if tuple(cfg["snap_times"]) == tuple(check_times(cfg)):  # the same run serves the spectra
+                _FINE_CACHE[_fine_key(cfg, seed, cfg["snap_times"])] = (
+                    [energy_spectrum(uniform_velocity(st, nf), cfg["L"]) for st in res["states"]],
+                    dict(wall=res["wall"], steps=res["steps"]))
             for st, ok in zip(res["states"], finite):
                 if not ok:
                     continue
@@ -335,12 +351,16 @@ def check_times(cfg):
 _FINE_CACHE = {}
 
 
+def _fine_key(cfg, seed, times):
+    return (cfg["fine"], cfg["L"], viscosity(cfg), seed, cfg["n_modes"], cfg["amplitude"], cfg["kmax"], cfg["flux"],
+            tuple(float(t) for t in times))
+
+
 def fine_spectra(cfg, seed, times):
     """Spectra of the fine run at `times` (kept for the process: calibration
     and evaluation of several models share the fine runs of a seed)."""
     nf = cfg["fine"]
-    key = (nf, cfg["L"], viscosity(cfg), seed, cfg["n_modes"], cfg["amplitude"], cfg["kmax"], cfg["flux"],
-           tuple(times))
+    key = _fine_key(cfg, seed, times)
     if key not in _FINE_CACHE:
         fine = run(nf, cfg["L"], viscosity(cfg), seed, cfg, times)
         specs = [energy_spectrum(uniform_velocity(st, nf), cfg["L"]) for st in fine["states"]]
@@ -448,12 +468,17 @@ def main(argv=None):
     ap.add_argument("--constant-multipliers", type=float, nargs="+", default=DEFAULT["constant_multipliers"],
                     help="candidate multipliers of the constant-coefficient baseline (calibrated the same way)")
     ap.add_argument("--no-constant", action="store_true", help="skip the constant-coefficient baseline")
+    ap.add_argument("--constant-multiplier", type=float, default=None,
+                    help="multiplier of the constant-coefficient baseline (default: calibrated)")
+    ap.add_argument("--no-eval", action="store_true",
+                    help="stop after training and calibration (the evaluation runs can follow with --weights)")
     args = ap.parse_args(argv)
     cfg = dict(DEFAULT, fine=args.fine, ratio=args.ratio, mid=args.mid, train_seeds=list(range(args.seeds)),
                epochs=args.epochs, snap_times=args.snap_times, eval_time=args.eval_time, flux=args.flux,
                kmax=args.kmax, multipliers=list(args.multipliers), calib_objective=args.calib_objective)
     os.makedirs(args.out, exist_ok=True)
     log = lambda s: print(s, flush=True)
+    PROGRESS["log"] = log
     report = {"config": cfg}
     if args.weights:
         w = np.load(args.weights)
@@ -491,9 +516,17 @@ def main(argv=None):
     const = None
     if not args.no_constant:
         const = constant_model(base, wts["shift"], wts["scale"], wts["lo"], wts["hi"])
-        cmult, ctable = calibrate(cfg, const, base, log, multipliers=list(args.constant_multipliers))
+        if args.constant_multiplier is None:
+            cmult, ctable = calibrate(cfg, const, base, log, multipliers=list(args.constant_multipliers))
+        else:
+            cmult, ctable = args.constant_multiplier, None
+            const.set_output_scale(base * cmult)
         report["constant_baseline"] = dict(multiplier=cmult, trials=ctable)
         log(f"constant-coefficient baseline: multiplier x{cmult}")
+    if args.no_eval:
+        with open(os.path.join(args.out, "closure_report.json"), "w") as f:
+            json.dump(report, f)
+        return
     report["evaluation"] = evaluate(cfg, mlp, log, constant=const)
     with open(os.path.join(args.out, "closure_report.json"), "w") as f:
         json.dump(report, f)
