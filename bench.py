@@ -67,10 +67,12 @@ def run_gpu(od, u0, p, steps, warmup):
     with od.Solver(p) as s:
         s.set_state(u0)
         dt = s.cfl_dt(CFL)
-        # per-kernel event timing first (3 steps, each launch bracketed by
-        # events), so that the GPU is busy right up to the warm-up steps
-        # instead of idling through the host-side CFL computation
-        kern = s.time_kernels(dt, 3)
+        # per-kernel event timing first (KERNEL_TIMING_STEPS steps, each
+        # launch bracketed by events), so that the GPU is busy right up to
+        # the warm-up steps: the first ~40 ms of work after an idle spell run
+        # ~4 % slower (clock ramp; 20-step windows measured back to back read
+        # 1.12 ms, then 1.073-1.076 ms per step at 48^3)
+        kern = s.time_kernels(dt, KERNEL_TIMING_STEPS)
         s.step(dt, warmup)
         s.synchronize()
         t0 = time.perf_counter()
@@ -97,6 +99,7 @@ def error_vs(ref, od, u0, p, dt, steps):
 
 
 PMC_PROFILE = "profiles/fr_step_kernels_48cubed_f32_pmc.txt"
+KERNEL_TIMING_STEPS = 30
 
 
 def traffic_from_profile(kernel):
@@ -326,6 +329,7 @@ def main():
         "traffic_source": f"{PMC_PROFILE}: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of {upd_name} in separate "
                           "profiling runs of this workload, copy-calibrated (not measured in this run)",
         "algorithmic_bytes_per_launch": alg_bytes_launch, "launch_ms": round(upd_ms, 5),
+        "launch_ms_over_steps": KERNEL_TIMING_STEPS,
         # the profiled HBM bytes per launch over the live launch time: how much
         # of the peak the kernel actually moves (the design's face traces and
         # viscous face fluxes on top of the compulsory state traffic)

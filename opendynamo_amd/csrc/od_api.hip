@@ -199,6 +199,10 @@ od_status capture_steps(od_ctx* c, double dt, uint32_t steps, hipGraphExec_t* ou
     e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    // the one-time device-side set-up of the graph now, not in its first
+    // replay (which otherwise runs ~1 ms long)
+    e = hipGraphUpload(*out, c->stream);
+    if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipGraphUpload: ") + hipGetErrorString(e));
     return OD_OK;
 }
 

@@ -253,6 +253,8 @@ od_status ensure_graph(od_batch* b) {
     e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    e = hipGraphUpload(b->graph, b->stream);  // device-side set-up now, not in the first replay
+    if (e != hipSuccess) return fail(OD_ERR_HIP, std::string("hipGraphUpload: ") + hipGetErrorString(e));
     return OD_OK;
 }
 
