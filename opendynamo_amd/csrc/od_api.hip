@@ -397,22 +397,11 @@ od_status od_get_state_async(od_ctx* c, uint64_t* ticket) {
     // Borrow a free pinned slot, blocking until a consumer returns one
     // (pool_rx.recv(), solver.rs:320): a slot is never refilled while its
     // previous snapshot is still being read.
-    int slot = -1;
     uint64_t t = 0;
-    {
-        std::unique_lock<std::mutex> lk(c->io_mu);
-        c->io_cv.wait(lk, [c] { return !c->slot_ticket[0] || !c->slot_ticket[1] || !c->slot_ticket[2]; });
-        slot = !c->slot_ticket[0] ? 0 : (!c->slot_ticket[1] ? 1 : 2);
-        t = c->next_ticket++;
-        c->slot_ticket[slot] = t;
-    }
+    const int slot = c->slots.lend(&t);
     // On an error below the slot goes back to the pool before returning.
-    auto give_back = [c, slot](od_status st) {
-        {
-            std::lock_guard<std::mutex> lk(c->io_mu);
-            c->slot_ticket[slot] = 0;
-        }
-        c->io_cv.notify_all();
+    auto give_back = [c, t](od_status st) {
+        c->slots.give_back(t);
         return st;
     };
     const size_t bytes = (size_t)5 * c->NP * c->ubytes;
@@ -444,22 +433,9 @@ od_status od_get_state_async(od_ctx* c, uint64_t* ticket) {
     return OD_OK;
 }
 
-namespace {
-// Slot loaned to `ticket`, or -1.  Caller holds io_mu.
-int loaned_slot(const od_ctx* c, uint64_t ticket) {
-    for (int q = 0; q < 3; ++q)
-        if (ticket != 0 && c->slot_ticket[q] == ticket) return q;
-    return -1;
-}
-}  // namespace
-
 od_status od_wait(od_ctx* c, uint64_t ticket, const void** host_state) {
     if (!c) return fail(OD_ERR_INVALID, "od_wait: null context");
-    int slot;
-    {
-        std::lock_guard<std::mutex> lk(c->io_mu);
-        slot = loaned_slot(c, ticket);
-    }
+    const int slot = c->slots.slot_of(ticket);
     if (slot < 0) return fail(OD_ERR_STATE, "od_wait: unknown or already released ticket");
     HIP_TRY(hipEventSynchronize(c->slot_done[slot]));
     if (host_state) *host_state = c->pinned[slot];
@@ -468,13 +444,7 @@ od_status od_wait(od_ctx* c, uint64_t ticket, const void** host_state) {
 
 od_status od_release(od_ctx* c, uint64_t ticket) {
     if (!c) return fail(OD_ERR_INVALID, "od_release: null context");
-    {
-        std::lock_guard<std::mutex> lk(c->io_mu);
-        const int slot = loaned_slot(c, ticket);
-        if (slot < 0) return fail(OD_ERR_STATE, "od_release: unknown or already released ticket");
-        c->slot_ticket[slot] = 0;
-    }
-    c->io_cv.notify_all();
+    if (!c->slots.give_back(ticket)) return fail(OD_ERR_STATE, "od_release: unknown or already released ticket");
     return OD_OK;
 }
 

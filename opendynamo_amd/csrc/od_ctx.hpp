@@ -3,10 +3,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <condition_variable>
-#include <mutex>
 
 #include "od_host.hpp"
+#include "od_slots.hpp"
 
 struct od_ctx {
     od_params p;
@@ -35,13 +34,9 @@ struct od_ctx {
     // device, in stream order) and the IO stream copies from here.
     void* staged[3] = {nullptr, nullptr, nullptr};
     hipEvent_t staged_done[3] = {nullptr, nullptr, nullptr};
-    // Pinned staging pool (solver.rs:163-175): a slot is loaned to a ticket by
-    // od_get_state_async and returned by od_release; slot_ticket 0 = free.
-    // Guarded by io_mu; io_cv signals a returned slot.
-    uint64_t next_ticket = 1;
-    uint64_t slot_ticket[3] = {0, 0, 0};
-    std::mutex io_mu;
-    std::condition_variable io_cv;
+    // Pinned staging pool (solver.rs:163-175): a slot is lent to a ticket by
+    // od_get_state_async and given back by od_release.
+    od::SlotPool<3> slots;
     size_t device_bytes = 0;
     // learned closure: per-element eddy viscosity [E] in the compute type
     void* mu_t = nullptr;
