@@ -57,6 +57,38 @@ def test_eddy_viscosity_step_matches_oracle(real):
         assert a.tobytes() == b.tobytes()
 
 
+@pytest.mark.parametrize("n", [4, 3], ids=["blocks_4cubed", "tiles_3cubed"])
+def test_fast_eddy_viscosity_step_against_fp64_oracle(n):
+    """The fast f32 build with a per-element eddy viscosity (the closure's
+    coupling) and a body force: 2x2x2-element block kernels at 4^3, two-element
+    tiles at 3^3, both within the fast build's tolerance of the FP64 oracle
+    (2e-5 of each variable's largest value, as for the other fast-math cases)."""
+    announce(test="mu_t_step_fast", n=n)
+    kw = dict(ns_kw(od.OD_F32, fast=1), viscosity_model=1, body_force_x=1e-3)
+    rng = np.random.default_rng(7)
+    mu_t = rng.uniform(0.0, 3e-3, n ** 3)
+    u0 = box_state(n, dtype=np.float32)
+    p = od.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **kw)
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        s.set_eddy_viscosity(mu_t)
+        s.step(0.01, 3)
+        got = s.get_state().astype(np.float64).reshape(5, -1)
+    okw = dict(kw, u_real=od.OD_F64)
+    okw.pop("fast_math")
+    po = O.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **okw)
+    want = O.step_mut(po, u0.astype(np.float64), mu_t, float(np.float32(0.01)), 3).reshape(5, -1)
+    err = np.abs(got - want).max(axis=1) / np.abs(want).max(axis=1)
+    print("fast f32 + mu_t vs FP64 oracle, rel err per var", err, flush=True)
+    assert np.all(err <= 2e-5), err
+    # the eddy viscosity acts: the same step without it differs
+    with od.Solver(p) as s:
+        s.set_state(u0)
+        s.step(0.01, 3)
+        plain = s.get_state().astype(np.float64).reshape(5, -1)
+    assert np.abs(plain - want).max() > 10 * np.abs(got - want).max()
+
+
 def test_zero_eddy_viscosity_is_plain_fr():
     n = 4
     p = od.params(n, n, n, TWO_PI, TWO_PI, TWO_PI, **ns_kw(od.OD_F32, fast=1))
