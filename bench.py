@@ -133,6 +133,37 @@ def hip_copy_calibration(nbytes):
             "launches": 3}
 
 
+def device_copy_rate(nbytes=1 << 30, reps=3):
+    """Bytes moved (read + written) per second by the HIP runtime's own
+    device-to-device copy of `nbytes`, event-timed on the null stream after
+    one untimed copy: what this part delivers to a plain streaming kernel,
+    the practical ceiling for a bandwidth-bound kernel's traffic."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    n = ctypes.c_size_t(nbytes)
+    if hip.hipMalloc(ctypes.byref(a), n) or hip.hipMalloc(ctypes.byref(b), n):
+        raise RuntimeError("hipMalloc failed in the copy-rate measurement")
+    try:
+        if hip.hipMemset(a, 0, n) or hip.hipEventCreate(ctypes.byref(e0)) or hip.hipEventCreate(ctypes.byref(e1)):
+            raise RuntimeError("copy-rate set-up failed")
+        hip.hipMemcpyAsync(b, a, n, 3, vp())  # hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e0, vp())
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, n, 3, vp())
+        hip.hipEventRecord(e1, vp())
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value / 1e3) / 1e9
+
+
 def host_cpu():
     """The host's CPUs as this process sees them: affinity count, logical CPU
     count, the CPU model string and the thread allotment of the environment
@@ -318,6 +349,7 @@ def main():
     alg_bytes_launch = ALG_BYTES_PER_DOF_STEP / 3.0 * dof
     achieved = alg_bytes_launch / (upd_ms / 1e3) / 1e9
     traffic = traffic_from_profile(upd_name)
+    copy_gbs = device_copy_rate()
     roofline = {
         "bound": "hbm", "kernel": upd_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
@@ -334,6 +366,10 @@ def main():
         # of the peak the kernel actually moves (the design's face traces and
         # viscous face fluxes on top of the compulsory state traffic)
         "traffic_frac": round(traffic / (upd_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5) if traffic else None,
+        # the same bytes against what a plain device copy reaches on this part
+        # (measured in this run, after the timed steps)
+        "device_copy_GBps": round(copy_gbs, 1),
+        "traffic_frac_of_copy": round(traffic / (upd_ms / 1e3) / 1e9 / copy_gbs, 4) if traffic else None,
         "step_frac_alg_hbm": round(value * 1e9 * ALG_BYTES_PER_DOF_STEP / 1e9 / HBM_PEAK_GBS, 5),
         "valu_frac_alg_flops": round(value * FLOP_PER_DOF_STEP / 1e3 / FP32_VECTOR_PEAK_TFLOPS, 5),
     }
