@@ -3,9 +3,9 @@
     python tests/golden/make_golden.py
 
 Small cases are stored as input/output vectors (golden_small.npz); the 8^3
-config-1 trajectory (periodic Euler, Rusanov, f64, CFL 0.3, 100 steps) is
-stored as SHA-256 digests of the state bytes after 1, 10 and 100 steps
-(golden_config1.json) since its IC is regenerated deterministically.
+periodic Euler trajectory (Rusanov, f64, CFL 0.3, 100 steps) is stored as
+SHA-256 digests of the state bytes after 1, 10 and 100 steps
+(golden_euler8.json) since its IC is regenerated deterministically.
 """
 import hashlib
 import json
@@ -37,7 +37,7 @@ SMALL = {
 }
 
 
-def config1():
+def euler8():
     L = float(np.float32(TWO_PI))
     p = O.params(8, 8, 8, L, L, L, u_real=8, flux="rusanov", viscous=False)
     u = ic.fourier_state(8, 8, 8, L, L, L, u0=(0.1, 0.0, 0.0), amplitude=1e-2, n_modes=8, seed=0)
@@ -57,7 +57,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "golden_small.npz"), **arrays)
     with open(os.path.join(HERE, "golden_small.json"), "w") as f:
         json.dump(meta, f, indent=1)
-    p, u = config1()
+    p, u = euler8()
     dt = O.cfl_dt(p, u, 0.3)
     digests = {"dt": dt, "sha256_in": hashlib.sha256(u.tobytes()).hexdigest()}
     cur, done = u, 0
@@ -66,7 +66,7 @@ def main():
         done = n
         digests[f"sha256_after_{n}"] = hashlib.sha256(cur.tobytes()).hexdigest()
         digests[f"sums_after_{n}"] = O.sums(p, cur).tolist()
-    with open(os.path.join(HERE, "golden_config1.json"), "w") as f:
+    with open(os.path.join(HERE, "golden_euler8.json"), "w") as f:
         json.dump(digests, f, indent=1)
     print("golden fixtures written")
 

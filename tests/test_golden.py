@@ -12,12 +12,12 @@ from oracle import oracle as O
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SMALL = json.load(open(os.path.join(HERE, "golden_small.json")))
 DATA = np.load(os.path.join(HERE, "golden_small.npz"))
-CONFIG1 = json.load(open(os.path.join(HERE, "golden_config1.json")))
+EULER8 = json.load(open(os.path.join(HERE, "golden_euler8.json")))
 
 
-def _config1():
-    from golden.make_golden import config1
-    return config1()
+def _euler8():
+    from golden.make_golden import euler8
+    return euler8()
 
 
 @pytest.mark.parametrize("name", sorted(SMALL))
@@ -28,12 +28,12 @@ def test_oracle_reproduces_small_golden(name):
     np.testing.assert_array_equal(out, DATA[name + "_out"])
 
 
-def test_oracle_reproduces_config1_digests():
-    p, u = _config1()
-    assert hashlib.sha256(u.tobytes()).hexdigest() == CONFIG1["sha256_in"]
-    assert O.cfl_dt(p, u, 0.3) == CONFIG1["dt"]
-    out = O.step(p, u, CONFIG1["dt"], 10)
-    assert hashlib.sha256(out.tobytes()).hexdigest() == CONFIG1["sha256_after_10"]
+def test_oracle_reproduces_euler8_digests():
+    p, u = _euler8()
+    assert hashlib.sha256(u.tobytes()).hexdigest() == EULER8["sha256_in"]
+    assert O.cfl_dt(p, u, 0.3) == EULER8["dt"]
+    out = O.step(p, u, EULER8["dt"], 10)
+    assert hashlib.sha256(out.tobytes()).hexdigest() == EULER8["sha256_after_10"]
 
 
 @pytest.mark.gpu
@@ -49,17 +49,17 @@ def test_gpu_reproduces_small_golden(name):
 
 
 @pytest.mark.gpu
-def test_gpu_reproduces_config1_digests():
+def test_gpu_reproduces_euler8_digests():
     import opendynamo_amd as od
-    p, u = _config1()
-    print("CASE golden config1 100 steps", flush=True)
+    p, u = _euler8()
+    print("CASE golden euler 8^3 100 steps", flush=True)
     gp = od.params(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, u_real=8, flux="rusanov", viscous=False)
     with od.Solver(gp) as s:
         s.set_state(u)
         done = 0
         for n in (1, 10, 100):
-            s.step(CONFIG1["dt"], n - done)
+            s.step(EULER8["dt"], n - done)
             done = n
             got = s.get_state()
-            assert hashlib.sha256(got.tobytes()).hexdigest() == CONFIG1[f"sha256_after_{n}"]
-            np.testing.assert_allclose(s.sums(), CONFIG1[f"sums_after_{n}"], rtol=1e-13, atol=1e-13)
+            assert hashlib.sha256(got.tobytes()).hexdigest() == EULER8[f"sha256_after_{n}"]
+            np.testing.assert_allclose(s.sums(), EULER8[f"sums_after_{n}"], rtol=1e-13, atol=1e-13)
