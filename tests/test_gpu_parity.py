@@ -191,11 +191,16 @@ def test_uniform_flow_free_stream():
 FAST_TOL = 2e-5  # max |gpu - oracle_f64| / max |oracle_f64| per variable
 
 
+# odd nx: two-element tiles; all dimensions even: 2x2x2-element block kernels
+FAST_DIMS = [(3, 4, 2), (4, 4, 2)]
+
+
+@pytest.mark.parametrize("dims", FAST_DIMS, ids=["tiles", "blocks"])
 @pytest.mark.parametrize("flux,wall,viscous,visc_model", CASES)
-def test_fast_math_against_fp64_oracle(flux, wall, viscous, visc_model):
-    nx, ny, nz = 3, 4, 2
+def test_fast_math_against_fp64_oracle(flux, wall, viscous, visc_model, dims):
+    nx, ny, nz = dims
     lx, ly, lz = TWO_PI, 2.0, np.pi
-    announce(test="fast", flux=flux, wall=wall, viscous=viscous, visc_model=visc_model)
+    announce(test="fast", dims=dims, flux=flux, wall=wall, viscous=viscous, visc_model=visc_model)
     pg = od.params(nx, ny, nz, lx, ly, lz, u_real=4, flux=flux, y_wall=wall, viscous=viscous,
                    viscosity_model=visc_model, mu_ref=2e-3, fast_math=1)
     po = O.params(nx, ny, nz, lx, ly, lz, u_real=8, flux=flux, y_wall=wall, viscous=viscous,
@@ -209,6 +214,31 @@ def test_fast_math_against_fp64_oracle(flux, wall, viscous, visc_model):
     err = rel_err_per_var(got, ref)
     print("fast-math rel err per var", err, flush=True)
     assert np.all(err < FAST_TOL), err
+
+
+FAST_RHS_TOL = 1e-4  # max |gpu - oracle_f64| / max |oracle_f64| per variable of the RHS
+
+
+@pytest.mark.parametrize("dims", FAST_DIMS, ids=["tiles", "blocks"])
+@pytest.mark.parametrize("flux,wall,viscous,visc_model", CASES)
+def test_fast_rhs_against_fp64_oracle(flux, wall, viscous, visc_model, dims):
+    """od_rhs of the fast build (the update kernels' RHS-only mode) against
+    the FP64 oracle's corrected RHS, tiles and blocks."""
+    nx, ny, nz = dims
+    lx, ly, lz = TWO_PI, 2.0, np.pi
+    announce(test="fast rhs", dims=dims, flux=flux, wall=wall, viscous=viscous, visc_model=visc_model)
+    pg = od.params(nx, ny, nz, lx, ly, lz, u_real=4, flux=flux, y_wall=wall, viscous=viscous,
+                   viscosity_model=visc_model, mu_ref=2e-3, fast_math=1)
+    po = O.params(nx, ny, nz, lx, ly, lz, u_real=8, flux=flux, y_wall=wall, viscous=viscous,
+                  viscosity_model=visc_model, mu_ref=2e-3)
+    u = smooth_state(nx, ny, nz, lx, ly, lz, seed=3, amp=0.08, dtype=np.float32)
+    ref = O.rhs_total(po, u.astype(np.float64))
+    with od.Solver(pg) as s:
+        s.set_state(u)
+        got = s.rhs()
+    err = rel_err_per_var(got, ref)
+    print("fast-math rhs rel err per var", err, flush=True)
+    assert np.all(err < FAST_RHS_TOL), err
 
 
 def test_fast_math_deterministic_and_resume():
