@@ -1384,12 +1384,20 @@ struct Block {
     int32_t e0;             // its element index
 };
 
-template <class UT, class CT>
+// REVERSE: the work-group -> block map runs backwards (the last block of the
+// mesh first).  The visc kernel walks the mesh forwards and the update kernel
+// backwards, so each kernel starts where the previous one ended, on the data
+// that kernel touched last (its neighbour traces, viscous fluxes, state),
+// which is the most likely to still be in the memory-side cache.
+template <bool REVERSE, class UT, class CT>
 __device__ __forceinline__ Block make_block(const StageArgs<UT, CT>& a) {
     Block b;
-    b.ex0 = 2 * (int)blockIdx.x;
-    b.ey0 = 2 * (int)blockIdx.y;
-    b.ez0 = 2 * (int)blockIdx.z;
+    const int bx = REVERSE ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
+    const int by = REVERSE ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
+    const int bz = REVERSE ? (int)(gridDim.z - 1 - blockIdx.z) : (int)blockIdx.z;
+    b.ex0 = 2 * bx;
+    b.ey0 = 2 * by;
+    b.ez0 = 2 * bz;
     b.e0 = b.ex0 + a.nx * (b.ey0 + a.ny * b.ez0);
     return b;
 }
@@ -1564,7 +1572,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void o
     __shared__ BlockUpdateLds L;
     const int wave = wave_index(), pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Block b = make_block(a);
+    const Block b = make_block<true>(a);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t e = blk_elem(a, b, wave);
     const int64_t gbase = e * 64 + pt;
@@ -1797,7 +1805,7 @@ __global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, flo
     __shared__ BlockViscLds L;
     const int wave = wave_index(), pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Block b = make_block(a);
+    const Block b = make_block<false>(a);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t e = blk_elem(a, b, wave);
     const float cf = coef_load<float>((int)threadIdx.x);
