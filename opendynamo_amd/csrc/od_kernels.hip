@@ -613,6 +613,13 @@ template <class UT, class CT>
 __device__ __forceinline__ Tile make_tile(const StageArgs<UT, CT>& a) {
     return make_tile_at(a, 0, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
+// The same tiles in reverse order (the update kernels: they start where the
+// visc kernel ended, on the data it touched last).
+template <class UT, class CT>
+__device__ __forceinline__ Tile make_tile_reversed(const StageArgs<UT, CT>& a) {
+    return make_tile_at(a, 0, (int)(gridDim.x - 1 - blockIdx.x), (int)(gridDim.y - 1 - blockIdx.y),
+                        (int)(gridDim.z - 1 - blockIdx.z));
+}
 
 __device__ __forceinline__ bool tile_valid(const Tile& t, int le) { return le ? t.v1 : t.v0; }
 __device__ __forceinline__ int64_t tile_eidx(const Tile& t, int le) { return (int64_t)(le ? t.e1 : t.e0); }
@@ -1362,7 +1369,7 @@ template <class UT, class CT, bool VISC>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(kMinWavesUpdate<UT, CT>))) void od_update_kernel(
     StageArgs<UT, CT> a) {
     __shared__ UpdateLds<CT, VISC> L;
-    update_body<UT, CT, VISC>(a, make_tile(a), L, wave_index(), threadIdx.x & 63);
+    update_body<UT, CT, VISC>(a, make_tile_reversed(a), L, wave_index(), threadIdx.x & 63);
 }
 
 #if OD_FAST
@@ -1989,10 +1996,10 @@ __global__ __launch_bounds__(128 * PAIRS) __attribute__((amdgpu_waves_per_eu(LOO
     __shared__ UpdateLds<CT, VISC> L[PAIRS];
     const int pair = PAIRS == 1 ? 0 : (int)(threadIdx.x >> 7);
     const int wave = wave_index() & 1, pt = threadIdx.x & 63;
-    if constexpr (!LOOP) {
+    if constexpr (!LOOP) {  // units in reverse order, as the single-box update kernels
         StageArgs<UT, CT> ab;
         Tile tl;
-        batch_tile(a, b, (int)blockIdx.x * PAIRS + pair, ab, tl);
+        batch_tile(a, b, ((int)gridDim.x - 1 - (int)blockIdx.x) * PAIRS + pair, ab, tl);
         update_body<UT, CT, VISC>(ab, tl, L[pair], wave, pt);
         return;
     }
