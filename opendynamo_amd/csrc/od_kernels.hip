@@ -969,11 +969,23 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Streaming (non-temporal) accesses of the update kernels: the new state and
+// its traces are read next by the visc kernel, which is bound by instruction
+// issue rather than by memory, and the previous stage's state is not read
+// again in this stage.  Marked as streaming they do not displace, in the
+// memory-side cache, the viscous fluxes, state and traces that the visc
+// kernel has just touched and the update kernel is about to read.
+template <class T>
+__device__ __forceinline__ T stream_load(const T* p) { return __builtin_nontemporal_load(p); }
+template <class T>
+__device__ __forceinline__ void stream_store(T v, T* p) { __builtin_nontemporal_store(v, p); }
+
 // Face traces of one element (its state tile sUe in LDS) -> dst, the
 // element's kTraceStride block of tr_out.  The element's own wave writes
 // them: 16 lanes per face, 4 faces per pass.  Without BR1 (Euler) the strict
-// build skips the pairwise x traces, which only BR1 reads.
-template <class CT>
+// build skips the pairwise x traces, which only BR1 reads.  STREAM: streaming
+// stores (the update kernels).
+template <class CT, bool STREAM = false>
 __device__ __forceinline__ void write_traces_own(const CT (*sUe)[64], CT* dst, int lane, bool need_pairwise_x) {
     const int fp = lane & 15;
 #pragma unroll
@@ -985,7 +997,13 @@ __device__ __forceinline__ void write_traces_own(const CT (*sUe)[64], CT* dst, i
         const Coef4<CT> c = side_coef<CT>(f & 1);
         const int order = trace_face_order(f);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) dst[f * 80 + v * 16 + fp] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, order);
+        for (int v = 0; v < 5; ++v) {
+            const CT tv = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, order);
+            if constexpr (STREAM)
+                stream_store(tv, &dst[f * 80 + v * 16 + fp]);
+            else
+                dst[f * 80 + v * 16 + fp] = tv;
+        }
     }
 }
 
@@ -1299,7 +1317,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
         for (int v = 0; v < 5; ++v) uprv[v] = ucr[v];
     } else {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) uprv[v] = mine ? a.uprev[v * NP + gbase] : (UT)1;
+        for (int v = 0; v < 5; ++v) uprv[v] = mine ? stream_load(&a.uprev[v * NP + gbase]) : (UT)1;
     }
     UT unew[5];
 #pragma unroll
@@ -1328,7 +1346,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
             UT rr = (UT)rt;
             unew[v] = a.stage1 ? (UT)(a.ua * uprv[v] + a.uc * rr) : (UT)(a.ua * uprv[v] + a.ub * ucr[v] + a.uc * rr);
         }
-        if (mine) a.uout[v * NP + gbase] = unew[v];
+        if (mine) stream_store(unew[v], &a.uout[v * NP + gbase]);
     }
     if (a.mode == 1) return;
 
@@ -1338,7 +1356,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 #pragma unroll
     for (int v = 0; v < 5; ++v) L.sU[wave][v][pt] = (CT)unew[v];
     wave_sync();
-    if (mine) write_traces_own(L.sU[wave], a.tr_out + tile_eidx(tl, wave) * kTraceStride, pt, VISC);
+    if (mine) write_traces_own<CT, true>(L.sU[wave], a.tr_out + tile_eidx(tl, wave) * kTraceStride, pt, VISC);
 }
 
 template <class UT, class CT>
@@ -1463,7 +1481,8 @@ __device__ __forceinline__ void blk_write_ext_traces(const float (*sUe)[64], flo
     const Coef4<float> c = side_coef<float>(f & 1);
 #pragma unroll
     for (int v = 0; v < 5; ++v)
-        dst[f * 80 + v * 16 + fp] = trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, trace_face_order(f));
+        stream_store(trace4(sUe[v], base, stride, c.c0, c.c1, c.c2, c.c3, trace_face_order(f)),
+                     &dst[f * 80 + v * 16 + fp]);
 }
 
 // State tiles of a block's elements, one every 5 * 64 + 16 floats: the pad
@@ -1682,7 +1701,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void o
         for (int v = 0; v < 5; ++v) uprv[v] = ucr[v];
     } else {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) uprv[v] = a.uprev[v * NP + gbase];
+        for (int v = 0; v < 5; ++v) uprv[v] = stream_load(&a.uprev[v * NP + gbase]);
     }
     float unew[5];
 #pragma unroll
@@ -1703,7 +1722,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void o
             float rr = rt;
             unew[v] = a.stage1 ? (a.ua * uprv[v] + a.uc * rr) : (a.ua * uprv[v] + a.ub * ucr[v] + a.uc * rr);
         }
-        a.uout[v * NP + gbase] = unew[v];
+        stream_store(unew[v], &a.uout[v * NP + gbase]);
     }
     if (a.mode == 1) return;
 
@@ -1714,7 +1733,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void o
     if constexpr (EXT) {
         blk_write_ext_traces(L.sU[wave], a.tr_out + e * kTraceStride, wave, pt);
     } else {
-        write_traces_own(L.sU[wave], a.tr_out + e * kTraceStride, pt, VISC);
+        write_traces_own<float, true>(L.sU[wave], a.tr_out + e * kTraceStride, pt, VISC);
     }
 }
 
