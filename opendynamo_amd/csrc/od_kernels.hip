@@ -1012,13 +1012,18 @@ __device__ __forceinline__ void write_traces_own(const CT (*sUe)[64], CT* dst, i
 // (global_load_lds_dword: no registers; the data lands with the
 // neighbour-trace prefetch and is covered by the same wait); otherwise
 // through registers with the cast to the compute type.
-template <class UT, class CT>
+// Cache-policy operand of an LDS-DMA load: 0, or the non-temporal (streaming)
+// bit of gfx950's cache policy, for data read once.
+constexpr int kDmaDefault = 0;
+constexpr int kDmaStream = 2;
+
+template <class UT, class CT, int POLICY = kDmaDefault>
 __device__ __forceinline__ void load_state_tile(const UT* u, int64_t NP, int64_t e, CT (*sUe)[64], int pt) {
     if constexpr (std::is_same<UT, float>::value && std::is_same<CT, float>::value) {
 #pragma unroll
         for (int v = 0; v < 5; ++v)
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(u + v * NP + e * 64 + pt),
-                                             (__attribute__((address_space(3))) void*)&sUe[v][0], 4, 0, 0);
+                                             (__attribute__((address_space(3))) void*)&sUe[v][0], 4, 0, POLICY);
     } else {
 #pragma unroll
         for (int v = 0; v < 5; ++v) sUe[v][pt] = (CT)u[v * NP + e * 64 + pt];
@@ -1603,12 +1608,16 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void o
     const int64_t e = blk_elem(a, b, wave);
     const int64_t gbase = e * 64 + pt;
     const float cf = coef_load<float>((int)threadIdx.x);
-    if (pt < 60) {  // the element's three external neighbour traces: 60 16-byte LDS-DMA chunks
+    // the element's three external neighbour traces (60 16-byte LDS-DMA
+    // chunks) and its state, both read for the last time in this stage:
+    // streaming, like the update kernel's stores
+    if (pt < 60) {
         const float* src = blk_ext_trace_src(a, b, wave, pt);
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                         (__attribute__((address_space(3))) void*)&L.sJe[wave][0][0][0], 16, 0, 0);
+                                         (__attribute__((address_space(3))) void*)&L.sJe[wave][0][0][0], 16, 0,
+                                         kDmaStream);
     }
-    load_state_tile(a.uin, NP, e, L.sU[wave], pt);
+    load_state_tile<float, float, kDmaStream>(a.uin, NP, e, L.sU[wave], pt);
     coef_store<float>((int)threadIdx.x, cf);
     __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS copies have landed
     __syncthreads();
