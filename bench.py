@@ -355,9 +355,10 @@ def main():
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
         # the roofline is priced on HBM bytes; what limits the kernels as
         # measured (rocprofv3 FETCH/WRITE_SIZE and SQ_INSTS_VALU in the
-        # committed profile): the update kernel moves its bytes at the rate
-        # of a plain device copy, the visc kernel is VALU-issue bound
-        "limiter": "hbm (update kernel, at the device-copy rate) + valu (visc kernel)",
+        # committed profile): the update kernel's traffic arrives at or
+        # above the rate of a plain device copy (part of it from the
+        # memory-side cache), the visc kernel is VALU-issue bound
+        "limiter": "memory (update kernel, HBM + memory-side cache) + valu (visc kernel)",
         "traffic_source": f"{PMC_PROFILE}: rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of {upd_name} in separate "
                           "profiling runs of this workload, copy-calibrated (not measured in this run)",
         "algorithmic_bytes_per_launch": alg_bytes_launch, "launch_ms": round(upd_ms, 5),
