@@ -3,21 +3,18 @@
 // argv[1]), loads [physics].initial_state, saves the startup state, advances
 // n_steps RK3 steps on the GPU through libopendynamo_hip, and writes dumps and
 // checkpoints from background IO threads with the reference's labels.
-#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <deque>
-#include <memory>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/opendynamo.h"
 #include "config.hpp"
 #include "state_io.hpp"
+#include "writer_pool.hpp"
 
 namespace {
 
@@ -29,58 +26,7 @@ const char* flux_debug_name(const std::string& s) {  // main.rs:17 prints the en
     return s == "slau2" ? "Slau2" : s == "ausm+" ? "AusmPlus" : "Rusanov";
 }
 
-// Background writers of the dumps (spawn_io_task, solver.rs:745-818).  Each
-// holds its pinned slot from od_wait until the file is written and then
-// returns it with od_release, on every path, so the step loop's next
-// od_get_state_async never refills a buffer that is still being read.  The
-// destructor joins every writer, so writers finish before the context they
-// use is destroyed, also when an exception leaves the step loop.
-class WriterPool {
-   public:
-    template <class F>
-    void spawn(od_ctx* ctx, uint64_t ticket, F write) {
-        reap();
-        auto done = std::make_shared<std::atomic<bool>>(false);
-        writers_.push_back({std::thread([ctx, ticket, write, done]() {
-                                try {
-                                    const void* host = nullptr;
-                                    od_check(od_wait(ctx, ticket, &host), "od_wait");
-                                    write(host);
-                                } catch (const std::exception& e) {
-                                    std::fprintf(stderr, "IO task failed: %s\n", e.what());
-                                }
-                                if (od_release(ctx, ticket) != OD_OK)
-                                    std::fprintf(stderr, "IO task failed: od_release: %s\n", od_last_error());
-                                done->store(true, std::memory_order_release);
-                            }),
-                            done});
-    }
-    void join_all() {
-        for (auto& w : writers_)
-            if (w.th.joinable()) w.th.join();
-        writers_.clear();
-    }
-    ~WriterPool() { join_all(); }
 
-   private:
-    struct Writer {
-        std::thread th;
-        std::shared_ptr<std::atomic<bool>> done;
-    };
-    // join the writers that have finished; at most 3 are unfinished at a time
-    // (each holds a slot), so the list stays short
-    void reap() {
-        for (auto it = writers_.begin(); it != writers_.end();) {
-            if (it->done->load(std::memory_order_acquire)) {
-                it->th.join();
-                it = writers_.erase(it);
-            } else {
-                ++it;
-            }
-        }
-    }
-    std::deque<Writer> writers_;
-};
 
 // solver.rs:820-839: min/max of rho and u over the final state, in f32.
 template <class T>
@@ -167,7 +113,18 @@ int run(const std::string& cfg_path) {
         std::printf("dt from CFL %.3g: %.9g\n", *cfg.solver.cfl, (double)dt);
     }
     const uint64_t n_steps = cfg.solver.n_steps;
-    WriterPool writers;  // declared after `destroy`: joined before the context goes
+    // declared after `destroy`: the writers are joined before the context goes
+    odh::WriterPool writers(
+        [ctx](uint64_t ticket) {
+            const void* host = nullptr;
+            od_check(od_wait(ctx, ticket, &host), "od_wait");
+            return host;
+        },
+        [ctx](uint64_t ticket) {
+            if (od_release(ctx, ticket) == OD_OK) return true;
+            std::fprintf(stderr, "IO task failed: od_release: %s\n", od_last_error());
+            return false;
+        });
     std::printf("Starting RK3 explicit time integration...\n");
     const auto t0 = std::chrono::steady_clock::now();
     for (uint64_t step = 0; step < n_steps; ++step) {
@@ -180,7 +137,7 @@ int run(const std::string& cfg_path) {
             // labels count completed steps; t = base_t + (k+1) dt (solver.rs:769-810)
             const uint64_t label = base_step + step + 1;
             const double t = base_t + ((double)step + 1.0) * (double)dt;
-            writers.spawn(ctx, ticket, [=, &cfg](const void* host) {
+            writers.spawn(ticket, [=, &cfg](const void* host) {
                 odh::save_state_h5(label, t, kind, cfg.io.output_dir, cfg.io.prefix, host, cfg.mesh.nx, cfg.mesh.ny,
                                    cfg.mesh.nz, lx, ly, lz, comp);
                 if (want_ckpt)
