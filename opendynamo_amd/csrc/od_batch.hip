@@ -41,7 +41,8 @@ struct od_batch {
     od_schedule sched{0, 1, 0, 0};
     std::vector<double> table_dt;  // the dt values the device table holds
     int table_orient = -1;
-    int64_t n_tiles[3] = {0, 0, 0};
+    int64_t n_tiles[3] = {0, 0, 0};  // tiles of the boxes that run on tiles, per orientation
+    int64_t n_blocks = 0;            // 2x2x2-element blocks of the boxes that run on blocks
     hipGraphExec_t graph = nullptr;
     int cu_count = 0;
     size_t device_bytes = 0;
@@ -58,6 +59,21 @@ int64_t box_tiles(const od_box& x, int orient) {
 }
 
 bool f32_compute(const od_batch* b) { return b->p.compute_real == OD_F32; }
+
+// A box runs on 2x2x2-element blocks by the single-box rule (fast f32 state
+// and compute, all dimensions even; od_create), in batches whose box offsets
+// fit the kernel arguments; the other boxes run on two-element tiles.  Both
+// give the same bits, so every box still equals the same box run alone.
+bool block_box(const od_batch* b, const od_box& x) {
+    return b->p.fast_math && b->p.u_real == OD_F32 && b->p.compute_real == OD_F32 &&
+           b->boxes.size() <= (size_t)od::kInlineBoxes && x.nx % 2 == 0 && x.ny % 2 == 0 && x.nz % 2 == 0;
+}
+int64_t tile_count(const od_batch* b, const od_box& x, int orient) {
+    return block_box(b, x) ? 0 : box_tiles(x, orient);
+}
+int64_t block_count(const od_batch* b, const od_box& x) {
+    return block_box(b, x) ? (int64_t)(x.nx / 2) * (x.ny / 2) * (x.nz / 2) : 0;
+}
 
 // ---- per-box launches with the single-box kernels ---------------------------
 template <class UT, class CT>
@@ -83,6 +99,13 @@ int box_launch_t(od_batch* b, int i, int kind, int stage, double dt) {
         if constexpr (sizeof(CT) == 4)
             if (b->p.fast_math) return od::fastmath::launch_trace<UT, CT>(a, E, b->stream);
         return od::launch_trace<UT, CT>(a, E, b->stream);
+    }
+    if constexpr (sizeof(UT) == 4 && sizeof(CT) == 4) {
+        if (block_box(b, b->boxes[i])) {  // as od_step runs this box alone
+            if (kind == 0) return od::fastmath::launch_visc_block(a, b->stream);
+            return b->p.viscous ? od::fastmath::launch_update_block<true>(a, true, b->stream)
+                                : od::fastmath::launch_update_block<false>(a, true, b->stream);
+        }
     }
     if constexpr (sizeof(CT) == 4) {
         if (b->p.fast_math) {
@@ -125,7 +148,7 @@ od_status upload_table_t(od_batch* b, const double* dt, int orient) {
             d.cc[s] = stage_dt_coef<CT>(s, dt[i]);
             d.uc[s] = stage_dt_coef<UT>(s, dt[i]);
         }
-        tiles += box_tiles(x, orient);
+        tiles += tile_count(b, x, orient);
     }
     HIP_TRY(hipStreamSynchronize(b->stream));
     HIP_TRY(hipMemcpy(b->table, t.data(), t.size() * sizeof(t[0]), hipMemcpyHostToDevice));
@@ -179,7 +202,7 @@ int batch_launch_t(od_batch* b, int kind, int stage) {
         int64_t t = 0;
         for (size_t i = 0; i < b->boxes.size(); ++i) {
             ba.tile_off[i] = (int32_t)t;
-            t += box_tiles(b->boxes[i], s.orient);
+            t += tile_count(b, b->boxes[i], s.orient);
         }
     }
     int64_t g = ba.n_units;
@@ -199,6 +222,25 @@ int batch_launch_t(od_batch* b, int kind, int stage) {
     if (kind == 0) return od::launch_batch_visc<UT, CT>(a, ba, s.pairs, groups, b->stream);
     if (b->p.viscous) return od::launch_batch_update<UT, CT, true>(a, ba, s.pairs, groups, b->stream);
     return od::launch_batch_update<UT, CT, false>(a, ba, s.pairs, groups, b->stream);
+}
+
+// The block boxes' launch of phase `kind` (0 visc, 1 update).
+int block_launch(od_batch* b, int kind, int stage) {
+    od::StageArgs<float, float> a = make_stage_args<float, float>(b->p, 1, 1, 1, 1.0, 1.0, 1.0, b->u, b->tr, b->fv,
+                                                                  nullptr, stage, 0.0);
+    od::BlockBatch bb{};
+    bb.boxes = (const od::BoxDev<float, float>*)b->table;
+    bb.n_boxes = (int32_t)b->boxes.size();
+    bb.n_blocks = (int32_t)b->n_blocks;
+    bb.stage = stage;
+    int64_t t = 0;
+    for (size_t i = 0; i < b->boxes.size(); ++i) {
+        bb.blk_off[i] = (int32_t)t;
+        t += block_count(b, b->boxes[i]);
+    }
+    if (kind == 0) return od::fastmath::launch_batch_visc_block(a, bb, b->stream);
+    return b->p.viscous ? od::fastmath::launch_batch_update_block<true>(a, bb, b->stream)
+                        : od::fastmath::launch_batch_update_block<false>(a, bb, b->stream);
 }
 
 int batch_launch(od_batch* b, int kind, int stage) {
@@ -227,9 +269,16 @@ od_status enqueue_step(od_batch* b) {
             }
         return OD_OK;
     }
+    const bool tiles = b->n_tiles[b->sched.orient] > 0, blocks = b->n_blocks > 0;
     for (int s = 0; s < 3; ++s) {
-        if (b->p.viscous && batch_launch(b, 0, s)) return fail(OD_ERR_HIP, "od_batch_visc_kernel launch failed");
-        if (batch_launch(b, 1, s)) return fail(OD_ERR_HIP, "od_batch_update_kernel launch failed");
+        if (b->p.viscous && tiles && batch_launch(b, 0, s))
+            return fail(OD_ERR_HIP, "od_batch_visc_kernel launch failed");
+        if (b->p.viscous && blocks && block_launch(b, 0, s))
+            return fail(OD_ERR_HIP, "od_batch_visc_block_kernel launch failed");
+        // updates walk their boxes backwards: the block boxes' visc output is
+        // the most recent in the memory-side cache, so their update goes first
+        if (blocks && block_launch(b, 1, s)) return fail(OD_ERR_HIP, "od_batch_update_block_kernel launch failed");
+        if (tiles && batch_launch(b, 1, s)) return fail(OD_ERR_HIP, "od_batch_update_kernel launch failed");
     }
     return OD_OK;
 }
@@ -363,7 +412,10 @@ od_status od_batch_create(const od_params* p, const od_box* boxes, int32_t n_box
         off += (int64_t)boxes[i].nx * boxes[i].ny * boxes[i].nz;
     }
     for (int o = 0; o < 3; ++o)
-        for (int i = 0; i < n_boxes; ++i) b->n_tiles[o] += box_tiles(boxes[i], o);
+        for (int i = 0; i < n_boxes; ++i) b->n_tiles[o] += tile_count(b, boxes[i], o);
+    for (int i = 0; i < n_boxes; ++i) b->n_blocks += block_count(b, boxes[i]);
+    if (b->n_blocks * 512 > (int64_t)0xffffffff)
+        return cleanup(fail(OD_ERR_INVALID, "batch too large for one launch grid"));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     b->cu_count = prop.multiProcessorCount;
@@ -641,7 +693,8 @@ od_status od_batch_info(od_batch* b, double out[4]) {
     if (!b || !out) return fail(OD_ERR_INVALID, "od_batch_info: null argument");
     out[0] = 64.0 * (double)b->E;
     out[1] = (double)b->device_bytes;
-    out[2] = b->p.viscous ? 6.0 : 3.0;
+    const int kinds = (b->n_tiles[b->sched.orient] > 0 ? 1 : 0) + (b->n_blocks > 0 ? 1 : 0);
+    out[2] = (b->p.viscous ? 6.0 : 3.0) * kinds;
     out[3] = (double)b->boxes.size();
     return OD_OK;
 }

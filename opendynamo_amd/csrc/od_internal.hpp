@@ -87,6 +87,15 @@ struct BatchArgs {
     int32_t tile_off[kInlineBoxes];  // first tile of each box (n_boxes <= kInlineBoxes)
 };
 
+// The boxes of a batch that run on 2x2x2-element blocks (fast f32 build, all
+// dimensions even): blk_off[i] = the first block of box i in the batch block
+// numbering (a box without blocks: the next box's first block).
+struct BlockBatch {
+    const BoxDev<float, float>* boxes;
+    int32_t n_boxes, n_blocks, stage, pad;
+    int32_t blk_off[kInlineBoxes];
+};
+
 // Kernel launchers (explicitly instantiated for UT, CT in {float, double}).
 template <class UT, class CT>
 int launch_trace(const StageArgs<UT, CT>& a, int64_t n_elem, void* stream);
@@ -130,6 +139,10 @@ int upload_ops(const Ops<float>& f32, const Ops<double>& f64);
 template <bool VISC>
 int launch_update_block(const StageArgs<float, float>& a, bool ext_traces, void* stream);
 int launch_visc_block(const StageArgs<float, float>& a, void* stream);
+// The same kernels over every block box of a batch, one launch each.
+template <bool VISC>
+int launch_batch_update_block(const StageArgs<float, float>& a, const BlockBatch& bb, void* stream);
+int launch_batch_visc_block(const StageArgs<float, float>& a, const BlockBatch& bb, void* stream);
 }  // namespace fastmath
 
 }  // namespace od

@@ -1598,12 +1598,10 @@ __device__ __forceinline__ void blk_int_pass(const StageArgs<float, float>& a, c
 // EXT: write the new state's traces of the external faces only (the block
 // visc kernel forms the internal ones from its LDS state tiles).
 template <bool VISC, bool EXT>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void od_update_block_kernel(
-    StageArgs<float, float> a) {
+__device__ __forceinline__ void update_block_body(const StageArgs<float, float>& a, const Block& b) {
     __shared__ BlockUpdateLds L;
     const int wave = wave_index(), pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Block b = make_block<true>(a);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t e = blk_elem(a, b, wave);
     const int64_t gbase = e * 64 + pt;
@@ -1836,11 +1834,10 @@ __device__ __forceinline__ void blk_fv_pass(const StageArgs<float, float>& a, co
     }
 }
 
-__global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, float> a) {
+__device__ __forceinline__ void visc_block_body(const StageArgs<float, float>& a, const Block& b) {
     __shared__ BlockViscLds L;
     const int wave = wave_index(), pt = threadIdx.x & 63;
     const int i = pt & 3, j = (pt >> 2) & 3, k = pt >> 4;
-    const Block b = make_block<false>(a);
     const int64_t NP = (int64_t)a.nx * a.ny * a.nz * 64;
     const int64_t e = blk_elem(a, b, wave);
     const float cf = coef_load<float>((int)threadIdx.x);
@@ -1887,6 +1884,17 @@ __global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, flo
         blk_fv_pass<1>(a, b, L, wave - 4, pt);
     }
 }
+
+// One box: the update kernel walks the blocks backwards, the visc kernel
+// forwards (make_block).
+template <bool VISC, bool EXT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void od_update_block_kernel(
+    StageArgs<float, float> a) {
+    update_block_body<VISC, EXT>(a, make_block<true>(a));
+}
+__global__ __launch_bounds__(512) void od_visc_block_kernel(StageArgs<float, float> a) {
+    visc_block_body(a, make_block<false>(a));
+}
 #endif
 
 // ---------------------------------------------------------------------------
@@ -1928,6 +1936,32 @@ __device__ __forceinline__ Elem uni(const Elem& e) {
     return Elem{uni(e.xl), uni(e.xr), uni(e.yl), uni(e.yr), uni(e.zl), uni(e.zr)};
 }
 
+// Stage arguments of one box of a batch: the batch's arguments with the
+// box's dimensions, Jacobians, stage coefficients and buffer offsets.
+template <class UT, class CT>
+__device__ __forceinline__ void batch_box_args(const StageArgs<UT, CT>& a, const BoxDev<UT, CT>& bx, int stage,
+                                               StageArgs<UT, CT>& ab) {
+    ab = a;
+    ab.nx = uni(bx.nx);
+    ab.ny = uni(bx.ny);
+    ab.nz = uni(bx.nz);
+    ab.jx = uni(bx.jx);
+    ab.jy = uni(bx.jy);
+    ab.jz = uni(bx.jz);
+    ab.cc = uni(bx.cc[stage]);
+    ab.uc = uni(bx.uc[stage]);
+    const int64_t e_off = uni(bx.e_off);
+    const int64_t so = e_off * 320;  // box state [5][E_box][64] starts at 5 * 64 * e_off
+    ab.uin = a.uin + so;
+    ab.uprev = a.uprev + so;
+    ab.uout = a.uout + so;
+    ab.dbg = a.dbg ? a.dbg + so : a.dbg;
+    ab.tr_in = a.tr_in + e_off * kTraceStride;
+    ab.tr_out = a.tr_out + e_off * kTraceStride;
+    ab.fv = a.fv ? a.fv + e_off * kFvStride : a.fv;
+    ab.mu_t = a.mu_t ? a.mu_t + e_off : a.mu_t;
+}
+
 // Stage arguments and tile of batch tile t (t >= n_tiles: an empty tile).
 template <class UT, class CT>
 __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int t,
@@ -1953,26 +1987,8 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
         }
     }
     const BoxDev<UT, CT>& bx = b.boxes[lo];
-    ab = a;
-    ab.nx = uni(bx.nx);
-    ab.ny = uni(bx.ny);
-    ab.nz = uni(bx.nz);
-    ab.jx = uni(bx.jx);
-    ab.jy = uni(bx.jy);
-    ab.jz = uni(bx.jz);
-    ab.cc = uni(bx.cc[b.stage]);
-    ab.uc = uni(bx.uc[b.stage]);
-    const int64_t e_off = uni(bx.e_off);
+    batch_box_args(a, bx, b.stage, ab);
     const int tile_off = uni(bx.tile_off);
-    const int64_t so = e_off * 320;  // box state [5][E_box][64] starts at 5 * 64 * e_off
-    ab.uin = a.uin + so;
-    ab.uprev = a.uprev + so;
-    ab.uout = a.uout + so;
-    ab.dbg = a.dbg ? a.dbg + so : a.dbg;
-    ab.tr_in = a.tr_in + e_off * kTraceStride;
-    ab.tr_out = a.tr_out + e_off * kTraceStride;
-    ab.fv = a.fv ? a.fv + e_off * kFvStride : a.fv;
-    ab.mu_t = a.mu_t ? a.mu_t + e_off : a.mu_t;
     const int lt = tt - tile_off;
     const int d0 = b.orient == 0 ? (ab.nx + kEpb - 1) / kEpb : ab.nx;
     const int d1 = b.orient == 1 ? (ab.ny + kEpb - 1) / kEpb : ab.ny;
@@ -1986,6 +2002,52 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
     tl.v0 = ok;
     tl.v1 = ok && tl.v1;
 }
+
+#if OD_FAST
+// Ensembles on 2x2x2-element blocks: the boxes whose dimensions are all even
+// run the block kernels (one launch per phase for all of them), the others
+// the tile kernels.  Batch block t -> its box (the last whose first block is
+// <= t; a box without blocks shares the next box's first block, so it is
+// never the last such box) -> the box's stage arguments and the block within
+// the box, x fastest.  The update kernel walks the blocks backwards, as the
+// single-box one does.
+template <bool REVERSE>
+__device__ __forceinline__ void batch_block(const StageArgs<float, float>& a, const BlockBatch& bb,
+                                            StageArgs<float, float>& ab, Block& b) {
+    const int t = REVERSE ? bb.n_blocks - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    int lo = 0, hi = bb.n_boxes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bb.blk_off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    batch_box_args(a, bb.boxes[lo], bb.stage, ab);
+    const int lb = t - bb.blk_off[lo];
+    const int gx = ab.nx >> 1, gy = ab.ny >> 1;
+    b.ex0 = 2 * (lb % gx);
+    b.ey0 = 2 * ((lb / gx) % gy);
+    b.ez0 = 2 * (lb / (gx * gy));
+    b.e0 = b.ex0 + ab.nx * (b.ey0 + ab.ny * b.ez0);
+}
+
+template <bool VISC>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) void od_batch_update_block_kernel(
+    StageArgs<float, float> a, BlockBatch bb) {
+    StageArgs<float, float> ab;
+    Block b;
+    batch_block<true>(a, bb, ab, b);
+    update_block_body<VISC, true>(ab, b);
+}
+
+__global__ __launch_bounds__(512) void od_batch_visc_block_kernel(StageArgs<float, float> a, BlockBatch bb) {
+    StageArgs<float, float> ab;
+    Block b;
+    batch_block<false>(a, bb, ab, b);
+    visc_block_body(ab, b);
+}
+#endif
 
 // LOOP = false: one unit per work-group (grid >= units, order 0); the
 // persistent loop is a separate instantiation, so that its loop-carried
@@ -2133,6 +2195,21 @@ int launch_visc_block(const StageArgs<float, float>& a, void* stream) {
     hipLaunchKernelGGL(od_visc_block_kernel, block_grid(a), dim3(64 * kBlk), 0, (hipStream_t)stream, a);
     return check_launch();
 }
+template <bool VISC>
+int launch_batch_update_block(const StageArgs<float, float>& a, const BlockBatch& bb, void* stream) {
+    if (bb.n_blocks <= 0) return 0;
+    hipLaunchKernelGGL((od_batch_update_block_kernel<VISC>), dim3((unsigned)bb.n_blocks), dim3(64 * kBlk), 0,
+                       (hipStream_t)stream, a, bb);
+    return check_launch();
+}
+int launch_batch_visc_block(const StageArgs<float, float>& a, const BlockBatch& bb, void* stream) {
+    if (bb.n_blocks <= 0) return 0;
+    hipLaunchKernelGGL(od_batch_visc_block_kernel, dim3((unsigned)bb.n_blocks), dim3(64 * kBlk), 0,
+                       (hipStream_t)stream, a, bb);
+    return check_launch();
+}
+template int launch_batch_update_block<true>(const StageArgs<float, float>&, const BlockBatch&, void*);
+template int launch_batch_update_block<false>(const StageArgs<float, float>&, const BlockBatch&, void*);
 #endif
 
 template <class UT, class CT>

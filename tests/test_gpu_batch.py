@@ -171,3 +171,72 @@ def test_changing_dt_between_steps(order):
         b.step(dt2, 1)
         for q in range(len(BOXES)):
             assert b.get_state(q).tobytes() == ref[q].tobytes()
+
+
+# Boxes whose dimensions are all even run the 2x2x2-element block kernels of a
+# lone box in fast f32 batches; the rest run tiles.  Block boxes come first,
+# last and between tile boxes, so the block -> box search meets boxes without
+# blocks on either side.
+MIXED_BOXES = [(4, 2, 2, 4.0, 2.0, 2.0), (3, 3, 2, 3.0, 3.0, 2.0), (2, 4, 6, 2.0, 4.0, 6.0),
+               (6, 2, 4, 6.0, 2.0, 4.0), (5, 2, 3, 5.0, 2.0, 3.0), (4, 4, 4, 4.0, 4.0, 4.0)]
+MIXED_DT = [0.004, 0.003, 0.0025, 0.0035, 0.003, 0.002]
+MIXED_VARIANTS = {
+    "ns_wall": VARIANTS["f32_fast_ns_wall"],
+    "euler": dict(u_real=od.OD_F32, flux="rusanov", viscous=False, body_force_x=0.0, fast_math=1),
+}
+
+
+@pytest.mark.parametrize("variant", list(MIXED_VARIANTS))
+def test_block_and_tile_boxes_equal_boxes_run_alone(variant):
+    kw = MIXED_VARIANTS[variant]
+    n_steps = 3
+
+    def init(q, dtype):
+        nx, ny, nz, lx, ly, lz = MIXED_BOXES[q]
+        return smooth_state(nx, ny, nz, lx, ly, lz, seed=40 + q, amp=0.04, dtype=dtype)
+
+    ref = []
+    for q, (nx, ny, nz, lx, ly, lz) in enumerate(MIXED_BOXES):
+        with od.Solver(od.params(nx, ny, nz, lx, ly, lz, **kw)) as s:
+            s.set_state(init(q, s.dtype))
+            s.step(MIXED_DT[q], n_steps)
+            ref.append(s.get_state())
+    with od.Batch(batch_params(kw), MIXED_BOXES) as b:
+        per_phase = 2 if kw.get("viscous", True) else 1
+        assert b.info()["launches_per_step"] == 3 * per_phase * 2  # tile and block launches
+        for sched in SCHEDULES:
+            announce(test="block_and_tile_boxes", variant=variant, **sched)
+            b.set_schedule(**sched)
+            for q in range(len(MIXED_BOXES)):
+                b.set_state(q, init(q, b.dtype))
+            b.step(MIXED_DT, n_steps)
+            for q in range(len(MIXED_BOXES)):
+                assert b.get_state(q).tobytes() == ref[q].tobytes(), (variant, sched, q)
+        announce(test="block_and_tile_boxes_per_box", variant=variant)
+        for q in range(len(MIXED_BOXES)):
+            b.set_state(q, init(q, b.dtype))
+        b.time_steps(MIXED_DT, n_steps, per_box=True)
+        for q in range(len(MIXED_BOXES)):
+            assert b.get_state(q).tobytes() == ref[q].tobytes(), (variant, "per box", q)
+
+
+def test_all_block_boxes():
+    """A batch of even boxes only: no tile launch at all."""
+    kw = VARIANTS["f32_fast_ns_wall"]
+    boxes = [MIXED_BOXES[0], MIXED_BOXES[2], MIXED_BOXES[5]]
+    dts = [MIXED_DT[0], MIXED_DT[2], MIXED_DT[5]]
+    inits = [smooth_state(*x, seed=60 + q, amp=0.04, dtype=np.float32) for q, x in enumerate(boxes)]
+    ref = []
+    for q, x in enumerate(boxes):
+        with od.Solver(od.params(*x, **kw)) as s:
+            s.set_state(inits[q])
+            s.step(dts[q], 2)
+            ref.append(s.get_state())
+    announce(test="all_block_boxes")
+    with od.Batch(batch_params(kw), boxes) as b:
+        assert b.info()["launches_per_step"] == 6
+        for q in range(len(boxes)):
+            b.set_state(q, inits[q])
+        b.step(dts, 2)
+        for q in range(len(boxes)):
+            assert b.get_state(q).tobytes() == ref[q].tobytes(), q
