@@ -156,10 +156,12 @@ const char* od_last_error(void);
  * wall-bounded boxes of different sizes sharing one od_params (physics,
  * scheme, precision; its nx..lz are ignored).  Every RK3 stage advances all
  * boxes with one launch of the gradient kernel and one of the update kernel
- * (a grid walking the tiles of every box), or, with schedule order 2, with
- * per-box launches; either way one hipGraph per step.  Each box's result is
- * byte-identical to the same box run alone through od_step with the same
- * dt, for every schedule. */
+ * per kernel form (a grid walking the tiles of every box; in fast f32
+ * batches of up to 64 boxes, a second pair walking the 2x2x2-element blocks
+ * of the boxes whose dimensions are all even, the kernels od_step takes for
+ * such a box), or, with schedule order 2, with per-box launches; either way
+ * one hipGraph per step.  Each box's result is byte-identical to the same
+ * box run alone through od_step with the same dt, for every schedule. */
 typedef struct od_batch od_batch;
 
 typedef struct od_box {
@@ -168,7 +170,8 @@ typedef struct od_box {
 } od_box;
 
 /* How the batch kernels map tiles (two neighbouring elements, 128 threads)
- * to work-groups.  The autotuner searches these; results never depend on it. */
+ * to work-groups; block boxes always take one 512-thread work-group per
+ * block.  The autotuner searches these; results never depend on it. */
 typedef struct od_schedule {
     int32_t orient;            /* tile = element pair along 0 x, 1 y, 2 z */
     int32_t pairs;             /* tiles per work-group: 1 (128 threads) or 2 (256) */
@@ -210,11 +213,12 @@ od_status od_batch_autotune(od_batch* b, const double* dt, uint32_t steps_per_tr
 
 /* Time n_steps graph launches (ms total); per_box != 0 times the same steps
  * launched box by box with the single-box kernels instead (6 launches per
- * box per step, no graph), for comparison. */
+ * box per step, 3 without viscosity, no graph), for comparison. */
 od_status od_batch_time_steps(od_batch* b, const double* dt, uint32_t n_steps, int32_t per_box, double* ms_total);
 
 /* out[0] = total DOF, out[1] = device bytes held, out[2] = kernel launches
- * per step, out[3] = number of boxes. */
+ * per step of the batched schedules (2 or 1 per stage and kernel form, the
+ * forms being tiles and blocks), out[3] = number of boxes. */
 od_status od_batch_info(od_batch* b, double out[4]);
 
 /* ---- Learned closure ---------------------------------------------------
