@@ -1821,8 +1821,9 @@ __device__ __forceinline__ void blk_fv_pass(const StageArgs<float, float>& a, co
     const int64_t e = blk_elem(a, b, le);
     const FvQuad<float> f = fv_value<D>(a, L.sU[le], L.sG()[le], e, side, fp);
     FvQuad<float> g;
+    const int from = (pt ^ 48) << 2;  // lane 16 + fp <-> 32 + fp (byte address of the source lane)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) g.c[c] = __shfl_xor(f.c[c], 48);  // lane 16 + fp <-> 32 + fp
+    for (int c = 0; c < 4; ++c) g.c[c] = __int_as_float(__builtin_amdgcn_ds_bpermute(from, __float_as_int(f.c[c])));
     FvQuad<float>* dst = reinterpret_cast<FvQuad<float>*>(a.fv + e * kFvStride + (2 * D + side) * 64 + fp * 4);
     if (second == side) {
         *dst = f;  // external face
