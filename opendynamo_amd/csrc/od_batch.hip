@@ -41,8 +41,10 @@ struct od_batch {
     od_schedule sched{0, 1, 0, 0};
     std::vector<double> table_dt;  // the dt values the device table holds
     int table_orient = -1;
-    int64_t n_tiles[3] = {0, 0, 0};  // tiles of the boxes that run on tiles, per orientation
-    int64_t n_blocks = 0;            // 2x2x2-element blocks of the boxes that run on blocks
+    int64_t n_tiles[3] = {0, 0, 0};  // tiles (boxes on tiles, shells of the others), per orientation
+    int64_t n_blocks = 0;            // 2x2x2-element blocks of the boxes' cores
+    bool has_shell = false;          // some box has a block core and a tile shell
+    bool core_faces_stale = false;   // tr[0] lacks the block cores' internal faces (see before_steps)
     hipGraphExec_t graph = nullptr;
     int cu_count = 0;
     size_t device_bytes = 0;
@@ -60,19 +62,29 @@ int64_t box_tiles(const od_box& x, int orient) {
 
 bool f32_compute(const od_batch* b) { return b->p.compute_real == OD_F32; }
 
-// A box runs on 2x2x2-element blocks by the single-box rule (fast f32 state
-// and compute, all dimensions even; od_create), in batches whose box offsets
-// fit the kernel arguments; the other boxes run on two-element tiles.  Both
-// give the same bits, so every box still equals the same box run alone.
-bool block_box(const od_batch* b, const od_box& x) {
+// In fast f32 batches whose box offsets fit the kernel arguments, the core
+// of every box at least 2 elements wide in each direction runs on 2x2x2-
+// element blocks: the whole box when its dimensions are all even, else the
+// box less its last layer along each odd dimension, whose elements (the
+// shell) run on two-element tiles.  Blocks and tiles give the same bits for
+// every face, so every box still equals the same box run alone (on blocks
+// when all its dimensions are even, od_create's rule; on tiles otherwise).
+bool core_box(const od_batch* b, const od_box& x) {
     return b->p.fast_math && b->p.u_real == OD_F32 && b->p.compute_real == OD_F32 &&
-           b->boxes.size() <= (size_t)od::kInlineBoxes && x.nx % 2 == 0 && x.ny % 2 == 0 && x.nz % 2 == 0;
+           b->boxes.size() <= (size_t)od::kInlineBoxes && x.nx >= 2 && x.ny >= 2 && x.nz >= 2;
+}
+bool even_box(const od_box& x) { return x.nx % 2 == 0 && x.ny % 2 == 0 && x.nz % 2 == 0; }
+// Tiles of the shell (shell_tile's numbering): the x slab paired along y,
+// the rest of the y and z slabs paired along x.
+int64_t shell_tiles(const od_box& x) {
+    const int64_t hx = x.nx / 2, hy = (x.ny + 1) / 2, cy = x.ny & ~1;
+    return (x.nx % 2 ? hy * x.nz : 0) + (x.ny % 2 ? hx * x.nz : 0) + (x.nz % 2 ? hx * cy : 0);
 }
 int64_t tile_count(const od_batch* b, const od_box& x, int orient) {
-    return block_box(b, x) ? 0 : box_tiles(x, orient);
+    return core_box(b, x) ? shell_tiles(x) : box_tiles(x, orient);
 }
 int64_t block_count(const od_batch* b, const od_box& x) {
-    return block_box(b, x) ? (int64_t)(x.nx / 2) * (x.ny / 2) * (x.nz / 2) : 0;
+    return core_box(b, x) ? (int64_t)(x.nx / 2) * (x.ny / 2) * (x.nz / 2) : 0;
 }
 
 // ---- per-box launches with the single-box kernels ---------------------------
@@ -101,7 +113,7 @@ int box_launch_t(od_batch* b, int i, int kind, int stage, double dt) {
         return od::launch_trace<UT, CT>(a, E, b->stream);
     }
     if constexpr (sizeof(UT) == 4 && sizeof(CT) == 4) {
-        if (block_box(b, b->boxes[i])) {  // as od_step runs this box alone
+        if (core_box(b, b->boxes[i]) && even_box(b->boxes[i])) {  // as od_step runs this box alone
             if (kind == 0) return od::fastmath::launch_visc_block(a, b->stream);
             return b->p.viscous ? od::fastmath::launch_update_block<true>(a, true, b->stream)
                                 : od::fastmath::launch_update_block<false>(a, true, b->stream);
@@ -141,6 +153,7 @@ od_status upload_table_t(od_batch* b, const double* dt, int orient) {
         d.nx = x.nx;
         d.ny = x.ny;
         d.nz = x.nz;
+        d.shell = core_box(b, x) ? 1 : 0;
         d.jx = jacobian<CT>(x.lx, x.nx);
         d.jy = jacobian<CT>(x.ly, x.ny);
         d.jz = jacobian<CT>(x.lz, x.nz);
@@ -329,7 +342,11 @@ od_status check_schedule(const od_schedule* s) {
     return OD_OK;
 }
 
+od_status before_steps(od_batch* b, bool per_box);
+
 od_status time_graph(od_batch* b, uint32_t n_steps, double* ms) {
+    od_status st = before_steps(b, b->sched.order == 2);
+    if (st != OD_OK) return st;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -349,6 +366,19 @@ od_status time_graph(od_batch* b, uint32_t n_steps, double* ms) {
 od_status retrace_all(od_batch* b) {
     for (size_t i = 0; i < b->boxes.size(); ++i)
         if (box_launch(b, (int)i, 2, 0, 0.0)) return fail(OD_ERR_HIP, "od_trace_kernel launch failed");
+    b->core_faces_stale = false;
+    return OD_OK;
+}
+
+// Before steps on the batch's stream.  Per-box launches (schedule order 2,
+// per-box timing) run a box with a shell on tiles alone, and tiles read the
+// traces of every face of tr[0]; batched steps leave the block cores'
+// internal faces out of it (the block kernels form those from the state), so
+// after batched steps they are formed again from u0 first.  The trace kernel
+// forms them with the arithmetic of the update kernels' trace stores.
+od_status before_steps(od_batch* b, bool per_box) {
+    if (per_box && b->core_faces_stale) return retrace_all(b);
+    if (!per_box && b->has_shell) b->core_faces_stale = true;
     return OD_OK;
 }
 
@@ -413,7 +443,10 @@ od_status od_batch_create(const od_params* p, const od_box* boxes, int32_t n_box
     }
     for (int o = 0; o < 3; ++o)
         for (int i = 0; i < n_boxes; ++i) b->n_tiles[o] += tile_count(b, boxes[i], o);
-    for (int i = 0; i < n_boxes; ++i) b->n_blocks += block_count(b, boxes[i]);
+    for (int i = 0; i < n_boxes; ++i) {
+        b->n_blocks += block_count(b, boxes[i]);
+        b->has_shell = b->has_shell || (core_box(b, boxes[i]) && !even_box(boxes[i]));
+    }
     if (b->n_blocks * 512 > (int64_t)0xffffffff)
         return cleanup(fail(OD_ERR_INVALID, "batch too large for one launch grid"));
     hipDeviceProp_t prop;
@@ -514,7 +547,9 @@ od_status od_batch_step(od_batch* b, const double* dt, uint32_t n_steps) {
     if (s == OD_OK) s = ready(b, "od_batch_step");
     if (s != OD_OK) return s;
     HIP_TRY(hipSetDevice(b->device));
-    if ((s = ensure_table(b, dt)) != OD_OK || (s = ensure_graph(b)) != OD_OK) return s;
+    if ((s = ensure_table(b, dt)) != OD_OK || (s = ensure_graph(b)) != OD_OK ||
+        (s = before_steps(b, b->sched.order == 2)) != OD_OK)
+        return s;
     for (uint32_t q = 0; q < n_steps; ++q) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
     return OD_OK;
 }
@@ -635,6 +670,7 @@ od_status od_batch_autotune(od_batch* b, const double* dt, uint32_t steps_per_tr
             (st = ensure_graph(b)) != OD_OK)
             return done(st);
         double ms = 0.0;
+        if ((st = before_steps(b, c.order == 2)) != OD_OK) return done(st);
         if (hipGraphLaunch(b->graph, b->stream) != hipSuccess)
             return done(fail(OD_ERR_HIP, "od_batch_autotune: graph launch failed"));
         if ((st = time_graph(b, steps_per_trial, &ms)) != OD_OK) return done(st);
@@ -669,6 +705,7 @@ od_status od_batch_time_steps(od_batch* b, const double* dt, uint32_t n_steps, i
         if ((st = ensure_table(b, dt)) != OD_OK || (st = ensure_graph(b)) != OD_OK) return st;
         return time_graph(b, n_steps, ms_total);
     }
+    if ((st = before_steps(b, true)) != OD_OK) return st;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

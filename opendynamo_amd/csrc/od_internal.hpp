@@ -69,7 +69,7 @@ struct BoxDev {
     int64_t e_off;     // first element of the box
     int32_t tile_off;  // first tile of the box in the batch tile numbering
     int32_t nx, ny, nz;
-    int32_t pad;
+    int32_t shell;     // 1: the box's core runs on 2x2x2 blocks, its tiles cover the rest (the shell)
     CT jx, jy, jz;     // 2 / (L / n) in the compute type
     CT cc[3];          // per-stage c * dt of the box in the compute type
     UT uc[3];          // and in the state type
@@ -87,9 +87,10 @@ struct BatchArgs {
     int32_t tile_off[kInlineBoxes];  // first tile of each box (n_boxes <= kInlineBoxes)
 };
 
-// The boxes of a batch that run on 2x2x2-element blocks (fast f32 build, all
-// dimensions even): blk_off[i] = the first block of box i in the batch block
-// numbering (a box without blocks: the next box's first block).
+// The boxes of a batch whose core runs on 2x2x2-element blocks (fast f32
+// build; the core is the box less the last layer along each odd dimension):
+// blk_off[i] = the first block of box i in the batch block numbering (a box
+// without blocks: the next box's first block).
 struct BlockBatch {
     const BoxDev<float, float>* boxes;
     int32_t n_boxes, n_blocks, stage, pad;

@@ -1963,6 +1963,24 @@ __device__ __forceinline__ void batch_box_args(const StageArgs<UT, CT>& a, const
     ab.mu_t = a.mu_t ? a.mu_t + e_off : a.mu_t;
 }
 
+// Tile lt of the shell of a box whose core runs on 2x2x2 blocks: the
+// elements with x = nx - 1 (odd nx), then y = ny - 1 (odd ny), then z = nz - 1
+// (odd nz), each element in the first of these slabs it lies in.  The x slab
+// pairs its elements along y, the others along x.  A tile's elements and
+// neighbours come from make_tile_at as for any tile, so each face is formed
+// as in a box run on tiles alone.
+template <class UT, class CT>
+__device__ __forceinline__ Tile shell_tile(const StageArgs<UT, CT>& a, int lt) {
+    const int hx = a.nx >> 1, hy = (a.ny + 1) >> 1;
+    const int nA = (a.nx & 1) ? hy * a.nz : 0;
+    const int nB = (a.ny & 1) ? hx * a.nz : 0;
+    if (lt < nA) return make_tile_at(a, 1, a.nx - 1, lt % hy, lt / hy);
+    lt -= nA;
+    if (lt < nB) return make_tile_at(a, 0, lt % hx, a.ny - 1, lt / hx);
+    lt -= nB;
+    return make_tile_at(a, 0, lt % hx, lt / hx, a.nz - 1);
+}
+
 // Stage arguments and tile of batch tile t (t >= n_tiles: an empty tile).
 template <class UT, class CT>
 __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const BatchArgs<UT, CT>& b, int t,
@@ -1991,10 +2009,14 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
     batch_box_args(a, bx, b.stage, ab);
     const int tile_off = uni(bx.tile_off);
     const int lt = tt - tile_off;
-    const int d0 = b.orient == 0 ? (ab.nx + kEpb - 1) / kEpb : ab.nx;
-    const int d1 = b.orient == 1 ? (ab.ny + kEpb - 1) / kEpb : ab.ny;
-    const int t0 = uni(lt % d0), r = uni(lt / d0);
-    tl = make_tile_at(ab, b.orient, t0, uni(r % d1), uni(r / d1));
+    if (uni(bx.shell)) {
+        tl = shell_tile(ab, lt);
+    } else {
+        const int d0 = b.orient == 0 ? (ab.nx + kEpb - 1) / kEpb : ab.nx;
+        const int d1 = b.orient == 1 ? (ab.ny + kEpb - 1) / kEpb : ab.ny;
+        const int t0 = uni(lt % d0), r = uni(lt / d0);
+        tl = make_tile_at(ab, b.orient, t0, uni(r % d1), uni(r / d1));
+    }
     tl.e0 = uni(tl.e0);
     tl.e1 = uni(tl.e1);
     tl.el0 = uni(tl.el0);
@@ -2005,9 +2027,14 @@ __device__ __forceinline__ void batch_tile(const StageArgs<UT, CT>& a, const Bat
 }
 
 #if OD_FAST
-// Ensembles on 2x2x2-element blocks: the boxes whose dimensions are all even
-// run the block kernels (one launch per phase for all of them), the others
-// the tile kernels.  Batch block t -> its box (the last whose first block is
+// Ensembles on 2x2x2-element blocks: the core of every box (the box less the
+// last layer along each odd dimension; the whole box when all are even) runs
+// the block kernels, one launch per phase for all boxes, and the shell the
+// tile kernels (shell_tile).  The blocks' neighbour map (blk_ext_nb) needs
+// no change: a core block's right neighbour is the next element unless the
+// block ends the box, and its left one across x = 0 (y = 0, z = 0) is the
+// last element of the box, a shell element when that dimension is odd.
+// Batch block t -> its box (the last whose first block is
 // <= t; a box without blocks shares the next box's first block, so it is
 // never the last such box) -> the box's stage arguments and the block within
 // the box, x fastest.  The update kernel walks the blocks backwards, as the

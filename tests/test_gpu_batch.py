@@ -240,3 +240,33 @@ def test_all_block_boxes():
         b.step(dts, 2)
         for q in range(len(boxes)):
             assert b.get_state(q).tobytes() == ref[q].tobytes(), q
+
+
+def test_switching_between_batched_and_per_box_steps():
+    """Batched steps leave the internal faces of the block cores out of the
+    traces; per-box steps run a box with a shell on tiles alone, which read
+    every face, so the batch forms them again first.  Steps alternate between
+    the batched graph, the per-box graph and per-box launches without a new
+    state in between, and every box must follow the box run alone."""
+    kw = MIXED_VARIANTS["ns_wall"]
+    inits = [smooth_state(*x, seed=80 + q, amp=0.04, dtype=np.float32) for q, x in enumerate(MIXED_BOXES)]
+    ref = []
+    for q, x in enumerate(MIXED_BOXES):
+        with od.Solver(od.params(*x, **kw)) as s:
+            s.set_state(inits[q])
+            s.step(MIXED_DT[q], 7)
+            ref.append(s.get_state())
+    announce(test="switch_batched_per_box")
+    with od.Batch(batch_params(kw), MIXED_BOXES) as b:
+        for q in range(len(MIXED_BOXES)):
+            b.set_state(q, inits[q])
+        b.set_schedule(orient=0, pairs=1, order=0, groups_per_cu=0)
+        b.step(MIXED_DT, 2)
+        b.set_schedule(orient=0, pairs=1, order=2, groups_per_cu=0)
+        b.step(MIXED_DT, 1)
+        b.set_schedule(orient=2, pairs=2, order=1, groups_per_cu=0)
+        b.step(MIXED_DT, 2)
+        b.time_steps(MIXED_DT, 1, per_box=True)
+        b.step(MIXED_DT, 1)
+        for q in range(len(MIXED_BOXES)):
+            assert b.get_state(q).tobytes() == ref[q].tobytes(), q
