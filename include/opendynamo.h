@@ -158,8 +158,9 @@ const char* od_last_error(void);
  * boxes with one launch of the gradient kernel and one of the update kernel
  * per kernel form (a grid walking the tiles of every box; in fast f32
  * batches of up to 64 boxes, a second pair walking the 2x2x2-element blocks
- * of the boxes whose dimensions are all even, the kernels od_step takes for
- * such a box), or, with schedule order 2, with per-box launches; either way
+ * of every box's core -- the box less its last layer along each odd
+ * dimension -- with the tiles left for those layers), or, with schedule
+ * order 2, with per-box launches; either way
  * one hipGraph per step.  Each box's result is byte-identical to the same
  * box run alone through od_step with the same dt, for every schedule. */
 typedef struct od_batch od_batch;
