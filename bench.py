@@ -167,7 +167,7 @@ def device_copy_rate(nbytes=1 << 30, reps=3):
 def host_cpu():
     """The host's CPUs as this process sees them: affinity count, logical CPU
     count, the CPU model string and the thread allotment of the environment
-    (OMP_NUM_THREADS; the GPU hosts give each GPU's jobs a share of 16)."""
+    (OMP_NUM_THREADS, which a shared host sets to each GPU's share of CPUs)."""
     try:
         affinity = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -416,7 +416,7 @@ def main():
             "per_thread_value": round(cpu_value / threads, 7),
             "host": hc,
             "threads_note": "threads = the CPUs this process may use, capped by OMP_NUM_THREADS when the "
-                            "environment sets it (the GPU host allots 16 CPUs to each GPU's jobs)",
+                            "environment sets it (a shared host's share of CPUs per GPU)",
         }
         out["speedup_vs_cpu"] = round(value / cpu_value, 1)
         # per-variable error of the f32 GPU pipelines against the FP64 oracle
