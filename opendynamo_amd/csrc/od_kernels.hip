@@ -95,6 +95,34 @@ __device__ __forceinline__ T sel3(T a0, T a1, T a2, int d) {
     return d == 0 ? a0 : (d == 1 ? a1 : a2);
 }
 
+// The RK stage blend kA u^n + kB u + kC r (stage 1: kA u^n + kC r; rk.cu:12-25).
+// The fast build writes the fusion out, fma(kC, r, fma(kB, u, kA u^n)),
+// so that every kernel form rounds it alike; the strict build rounds each
+// product and sum, left to right.
+template <class T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_fmaf(a, b, c);
+    else
+        return __builtin_fma(a, b, c);
+}
+template <class T>
+__device__ __forceinline__ T rk_blend1(T kA, T kC, T un, T r) {
+#if OD_FAST
+    return fma_t(kC, r, kA * un);
+#else
+    return kA * un + kC * r;
+#endif
+}
+template <class T>
+__device__ __forceinline__ T rk_blend(T kA, T kB, T kC, T un, T u, T r) {
+#if OD_FAST
+    return fma_t(kC, r, fma_t(kB, u, kA * un));
+#else
+    return kA * un + kB * u + kC * r;
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Point physics in the compute type C (device_functions.cuh:1-257).
 // Direction d = 0/1/2 is an argument; all three reference variants
@@ -1324,7 +1352,7 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
 #pragma unroll
         for (int v = 0; v < 5; ++v) uprv[v] = mine ? stream_load(&a.uprev[v * NP + gbase]) : (UT)1;
     }
-    UT unew[5];
+    CT rts[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const CT(*J)[5][16] = L.sJ()[wave];
@@ -1337,23 +1365,46 @@ __device__ __forceinline__ void update_body(const StageArgs<UT, CT>& a, const Ti
         CT cy = -(J[2][v][fpy] * gLj * a.jy) - (J[3][v][fpy] * gRj * a.jy);
         CT cz = -(J[4][v][fpz] * gLk * a.jz) - (J[5][v][fpz] * gRk * a.jz);
 #endif
-        CT rt = r[v] + (cx + cy + cz);
-        if (a.mode == 1) {
-            if (mine) a.dbg[v * NP + gbase] = rt;
-            unew[v] = ucr[v];
-            continue;
-        }
-        if (!a.accum_state) {
-            CT up = (CT)uprv[v], uc = (CT)ucr[v];
-            CT val = a.stage1 ? (a.ca * up + a.cc * rt) : (a.ca * up + a.cb * uc + a.cc * rt);
-            unew[v] = (UT)val;
-        } else {
-            UT rr = (UT)rt;
-            unew[v] = a.stage1 ? (UT)(a.ua * uprv[v] + a.uc * rr) : (UT)(a.ua * uprv[v] + a.ub * ucr[v] + a.uc * rr);
-        }
-        if (mine) stream_store(unew[v], &a.uout[v * NP + gbase]);
+        rts[v] = r[v] + (cx + cy + cz);
     }
-    if (a.mode == 1) return;
+    if (a.mode == 1) {  // the stage's RHS into dbg, state unchanged
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+            if (mine) a.dbg[v * NP + gbase] = rts[v];
+        return;
+    }
+    UT unew[5];
+    if constexpr (std::is_same<UT, CT>::value) {
+        // one type: the compute-type and the state-type blends differ only
+        // in their coefficients, chosen once; uniform branch on the stage
+        const CT kA = a.accum_state ? a.ua : a.ca;
+        const CT kB = a.accum_state ? a.ub : a.cb;
+        const CT kC = a.accum_state ? a.uc : a.cc;
+        if (a.stage1) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) unew[v] = rk_blend1(kA, kC, uprv[v], rts[v]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) unew[v] = rk_blend(kA, kB, kC, uprv[v], ucr[v], rts[v]);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            const CT rt = rts[v];
+            if (!a.accum_state) {
+                CT up = (CT)uprv[v], uc = (CT)ucr[v];
+                CT val = a.stage1 ? (a.ca * up + a.cc * rt) : (a.ca * up + a.cb * uc + a.cc * rt);
+                unew[v] = (UT)val;
+            } else {
+                UT rr = (UT)rt;
+                unew[v] =
+                    a.stage1 ? (UT)(a.ua * uprv[v] + a.uc * rr) : (UT)(a.ua * uprv[v] + a.ub * ucr[v] + a.uc * rr);
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
+        if (mine) stream_store(unew[v], &a.uout[v * NP + gbase]);
 
     // Face traces of the new state for the next stage, each wave those of its
     // own element: from the RK phase on only wave w reads sU[w], so no
@@ -1710,28 +1761,34 @@ __device__ __forceinline__ void update_block_body(const StageArgs<float, float>&
 #pragma unroll
         for (int v = 0; v < 5; ++v) uprv[v] = stream_load(&a.uprev[v * NP + gbase]);
     }
-    float unew[5];
+    float rt[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         float cx = -(Jx0[v * 16 + fpx] * gLi + Jx1[v * 16 + fpx] * gRi);
         float cy = -(Jy0[v * 16 + fpy] * gLj + Jy1[v * 16 + fpy] * gRj);
         float cz = -(Jz0[v * 16 + fpz] * gLk + Jz1[v * 16 + fpz] * gRk);
-        float rt = r[v] + (cx + cy + cz);
-        if (a.mode == 1) {
-            a.dbg[v * NP + gbase] = rt;
-            unew[v] = ucr[v];
-            continue;
-        }
-        if (!a.accum_state) {
-            float val = a.stage1 ? (a.ca * uprv[v] + a.cc * rt) : (a.ca * uprv[v] + a.cb * ucr[v] + a.cc * rt);
-            unew[v] = val;
-        } else {
-            float rr = rt;
-            unew[v] = a.stage1 ? (a.ua * uprv[v] + a.uc * rr) : (a.ua * uprv[v] + a.ub * ucr[v] + a.uc * rr);
-        }
-        stream_store(unew[v], &a.uout[v * NP + gbase]);
+        rt[v] = r[v] + (cx + cy + cz);
     }
-    if (a.mode == 1) return;
+    if (a.mode == 1) {  // the stage's RHS into dbg, state unchanged
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a.dbg[v * NP + gbase] = rt[v];
+        return;
+    }
+    // the blend in the compute or the state type: both float here, so the
+    // two differ only in their coefficients, chosen once (scalar selects)
+    const float kA = a.accum_state ? a.ua : a.ca;
+    const float kB = a.accum_state ? a.ub : a.cb;
+    const float kC = a.accum_state ? a.uc : a.cc;
+    float unew[5];
+    if (a.stage1) {  // uniform branch: each form computes only its own blend
+#pragma unroll
+        for (int v = 0; v < 5; ++v) unew[v] = rk_blend1(kA, kC, uprv[v], rt[v]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) unew[v] = rk_blend(kA, kB, kC, uprv[v], ucr[v], rt[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) stream_store(unew[v], &a.uout[v * NP + gbase]);
 
     // Face traces of the new state.
 #pragma unroll
