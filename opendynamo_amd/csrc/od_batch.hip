@@ -237,7 +237,7 @@ int batch_launch_t(od_batch* b, int kind, int stage) {
     return od::launch_batch_update<UT, CT, false>(a, ba, s.pairs, groups, b->stream);
 }
 
-// The block boxes' launch of phase `kind` (0 visc, 1 update).
+// The block cores' launch of phase `kind` (0 visc, 1 update), all boxes in one.
 int block_launch(od_batch* b, int kind, int stage) {
     od::StageArgs<float, float> a = make_stage_args<float, float>(b->p, 1, 1, 1, 1.0, 1.0, 1.0, b->u, b->tr, b->fv,
                                                                   nullptr, stage, 0.0);
@@ -288,8 +288,9 @@ od_status enqueue_step(od_batch* b) {
             return fail(OD_ERR_HIP, "od_batch_visc_kernel launch failed");
         if (b->p.viscous && blocks && block_launch(b, 0, s))
             return fail(OD_ERR_HIP, "od_batch_visc_block_kernel launch failed");
-        // updates walk their boxes backwards: the block boxes' visc output is
-        // the most recent in the memory-side cache, so their update goes first
+        // updates walk their blocks and tiles backwards: the block cores' visc
+        // output is the most recent in the memory-side cache, so their update
+        // goes first
         if (blocks && block_launch(b, 1, s)) return fail(OD_ERR_HIP, "od_batch_update_block_kernel launch failed");
         if (tiles && batch_launch(b, 1, s)) return fail(OD_ERR_HIP, "od_batch_update_kernel launch failed");
     }
