@@ -157,7 +157,7 @@ const char* od_last_error(void);
  * scheme, precision; its nx..lz are ignored).  Every RK3 stage advances all
  * boxes with one launch of the gradient kernel and one of the update kernel
  * per kernel form (a grid walking the tiles of every box; in fast f32
- * batches of up to 64 boxes, a second pair walking the 2x2x2-element blocks
+ * batches, a second pair walking the 2x2x2-element blocks
  * of every box's core -- the box less its last layer along each odd
  * dimension -- with the tiles left for those layers), or, with schedule
  * order 2, with per-box launches; either way

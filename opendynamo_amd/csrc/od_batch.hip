@@ -62,16 +62,15 @@ int64_t box_tiles(const od_box& x, int orient) {
 
 bool f32_compute(const od_batch* b) { return b->p.compute_real == OD_F32; }
 
-// In fast f32 batches whose box offsets fit the kernel arguments, the core
-// of every box at least 2 elements wide in each direction runs on 2x2x2-
-// element blocks: the whole box when its dimensions are all even, else the
+// In fast f32 batches the core of every box at least 2 elements wide in each
+// direction runs on 2x2x2-element blocks: the whole box when its dimensions are all even, else the
 // box less its last layer along each odd dimension, whose elements (the
 // shell) run on two-element tiles.  Blocks and tiles give the same bits for
 // every face, so every box still equals the same box run alone (on blocks
 // when all its dimensions are even, od_create's rule; on tiles otherwise).
 bool core_box(const od_batch* b, const od_box& x) {
-    return b->p.fast_math && b->p.u_real == OD_F32 && b->p.compute_real == OD_F32 &&
-           b->boxes.size() <= (size_t)od::kInlineBoxes && x.nx >= 2 && x.ny >= 2 && x.nz >= 2;
+    return b->p.fast_math && b->p.u_real == OD_F32 && b->p.compute_real == OD_F32 && x.nx >= 2 && x.ny >= 2 &&
+           x.nz >= 2;
 }
 bool even_box(const od_box& x) { return x.nx % 2 == 0 && x.ny % 2 == 0 && x.nz % 2 == 0; }
 // Tiles of the shell (shell_tile's numbering): the x slab paired along y,
@@ -143,7 +142,7 @@ int box_launch(od_batch* b, int i, int kind, int stage, double dt) {
 template <class UT, class CT>
 od_status upload_table_t(od_batch* b, const double* dt, int orient) {
     std::vector<od::BoxDev<UT, CT>> t(b->boxes.size());
-    int64_t tiles = 0;
+    int64_t tiles = 0, blocks = 0;
     for (size_t i = 0; i < b->boxes.size(); ++i) {
         const od_box& x = b->boxes[i];
         od::BoxDev<UT, CT>& d = t[i];
@@ -154,6 +153,8 @@ od_status upload_table_t(od_batch* b, const double* dt, int orient) {
         d.ny = x.ny;
         d.nz = x.nz;
         d.shell = core_box(b, x) ? 1 : 0;
+        d.blk_off = (int32_t)blocks;
+        blocks += block_count(b, x);
         d.jx = jacobian<CT>(x.lx, x.nx);
         d.jy = jacobian<CT>(x.ly, x.ny);
         d.jz = jacobian<CT>(x.lz, x.nz);
@@ -247,7 +248,7 @@ int block_launch(od_batch* b, int kind, int stage) {
     bb.n_blocks = (int32_t)b->n_blocks;
     bb.stage = stage;
     int64_t t = 0;
-    for (size_t i = 0; i < b->boxes.size(); ++i) {
+    for (size_t i = 0; i < b->boxes.size() && i < (size_t)od::kInlineBoxes; ++i) {
         bb.blk_off[i] = (int32_t)t;
         t += block_count(b, b->boxes[i]);
     }

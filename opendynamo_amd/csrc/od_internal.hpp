@@ -70,6 +70,7 @@ struct BoxDev {
     int32_t tile_off;  // first tile of the box in the batch tile numbering
     int32_t nx, ny, nz;
     int32_t shell;     // 1: the box's core runs on 2x2x2 blocks, its tiles cover the rest (the shell)
+    int32_t blk_off;   // first block of the box's core in the batch block numbering
     CT jx, jy, jz;     // 2 / (L / n) in the compute type
     CT cc[3];          // per-stage c * dt of the box in the compute type
     UT uc[3];          // and in the state type
@@ -90,7 +91,8 @@ struct BatchArgs {
 // The boxes of a batch whose core runs on 2x2x2-element blocks (fast f32
 // build; the core is the box less the last layer along each odd dimension):
 // blk_off[i] = the first block of box i in the batch block numbering (a box
-// without blocks: the next box's first block).
+// without blocks: the next box's first block), held in the kernel arguments
+// for batches of up to kInlineBoxes boxes and in the box table always.
 struct BlockBatch {
     const BoxDev<float, float>* boxes;
     int32_t n_boxes, n_blocks, stage, pad;

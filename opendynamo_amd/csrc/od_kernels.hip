@@ -2100,16 +2100,28 @@ template <bool REVERSE>
 __device__ __forceinline__ void batch_block(const StageArgs<float, float>& a, const BlockBatch& bb,
                                             StageArgs<float, float>& ab, Block& b) {
     const int t = REVERSE ? bb.n_blocks - 1 - (int)blockIdx.x : (int)blockIdx.x;
-    int lo = 0, hi = bb.n_boxes - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bb.blk_off[mid] <= t)
-            lo = mid;
-        else
-            hi = mid - 1;
+    int lo = 0, hi = bb.n_boxes - 1, off;
+    if (bb.n_boxes <= kInlineBoxes) {  // kernel-argument copy: scalar loads, no memory latency chain
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (bb.blk_off[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        off = bb.blk_off[lo];
+    } else {
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (uni(bb.boxes[mid].blk_off) <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        off = uni(bb.boxes[lo].blk_off);
     }
     batch_box_args(a, bb.boxes[lo], bb.stage, ab);
-    const int lb = t - bb.blk_off[lo];
+    const int lb = t - off;
     const int gx = ab.nx >> 1, gy = ab.ny >> 1;
     b.ex0 = 2 * (lb % gx);
     b.ey0 = 2 * ((lb / gx) % gy);

@@ -272,3 +272,28 @@ def test_switching_between_batched_and_per_box_steps():
         b.step(MIXED_DT, 1)
         for q in range(len(MIXED_BOXES)):
             assert b.get_state(q).tobytes() == ref[q].tobytes(), q
+
+
+def test_more_boxes_than_the_kernel_argument_table():
+    """Beyond 64 boxes the tile and block kernels find their box through the
+    device table instead of the kernel arguments; 70 boxes of five shapes,
+    each equal to its shape stepped alone."""
+    kw = MIXED_VARIANTS["ns_wall"]
+    shapes = [(2, 2, 2, 2.0, 2.0, 2.0), (3, 2, 2, 3.0, 2.0, 2.0), (2, 3, 3, 2.0, 3.0, 3.0), (4, 2, 2, 4.0, 2.0, 2.0),
+              (3, 3, 3, 3.0, 3.0, 3.0)]
+    dts = [0.003, 0.0025, 0.0035, 0.002, 0.003]
+    inits = [smooth_state(*x, seed=90 + q, amp=0.04, dtype=np.float32) for q, x in enumerate(shapes)]
+    ref = []
+    for q, x in enumerate(shapes):
+        with od.Solver(od.params(*x, **kw)) as s:
+            s.set_state(inits[q])
+            s.step(dts[q], 2)
+            ref.append(s.get_state())
+    boxes = [shapes[q % 5] for q in range(70)]
+    announce(test="seventy_boxes")
+    with od.Batch(batch_params(kw), boxes) as b:
+        for q in range(70):
+            b.set_state(q, inits[q % 5])
+        b.step([dts[q % 5] for q in range(70)], 2)
+        for q in range(70):
+            assert b.get_state(q).tobytes() == ref[q % 5].tobytes(), q
