@@ -1,8 +1,10 @@
 """Ensembles (od_batch_*): every box of a batch advanced with one launch per
-kernel and stage must equal the same box run alone through od_step, byte for
-byte, for every schedule the autotuner can pick; tuning leaves the state
-unchanged.  Boxes have odd and even sizes in every direction so that tiles
-with a missing second element occur for each tile orientation."""
+kernel form and stage must equal the same box run alone through od_step,
+byte for byte, for every schedule the autotuner can pick; tuning leaves the
+state unchanged.  Boxes have odd and even sizes in every direction so that
+tiles with a missing second element occur for each tile orientation (the
+strict, f64 and mixed variants run every box on tiles; the fast f32 ones run
+each box's core on 2x2x2-element blocks and its odd layers on tiles)."""
 import numpy as np
 import pytest
 
